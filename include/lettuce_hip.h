@@ -1,0 +1,132 @@
+/*
+ * lettuce_hip.h -- C ABI of the MI355X (gfx950) gauge-field library.
+ *
+ * The reference (timo-eichhorn/Lettuce) is a header-only C++ code whose "API" is
+ * templates over compile-time extents plus functors holding references to
+ * caller-owned fields.  This ABI replaces that surface for the hot path with
+ * opaque handles and runtime extents; the granularity is one sweep / pass /
+ * trajectory per call, never one link.  Each entry point cites the reference
+ * interface it replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative LT_E* code on failure;
+ *     lt_last_error() returns a message for the calling thread's last failure.
+ *   - Host buffers are in the reference AoS layout: link index
+ *     ((((t*Nx+x)*Ny+y)*Nz+z)*4+mu), 18 doubles per link, row-major, interleaved
+ *     re/im (LettuceGauge/lattice.hpp:366-382, defines.hpp:122).
+ *   - A context owns one HIP stream; all work of its fields is ordered on it.
+ *     Calls returning host scalars synchronise that stream; the others only
+ *     enqueue.  One host thread per context at a time.
+ *   - All arithmetic is FP64.
+ */
+#ifndef LETTUCE_HIP_H
+#define LETTUCE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LT_OK            0
+#define LT_EINVAL       -1   /* bad argument (shape, null pointer, schedule) */
+#define LT_EHIP         -2   /* HIP runtime error */
+#define LT_ENOMEM       -3   /* device allocation failed */
+#define LT_ESTATE       -4   /* object in the wrong state / mismatched shapes */
+#define LT_ENUMERIC     -5   /* non-finite or non-unitary links detected */
+
+typedef struct lt_ctx_s   *lt_ctx;
+typedef struct lt_field_s *lt_field;
+typedef struct lt_flow_s  *lt_flow;
+
+/* local-update schedules: Iterator::Checkerboard and Iterator::Checkerboard4
+ * (LettuceGauge/iterators/iterators.hpp:65-87, 141-169) */
+#define LT_CB2 2
+#define LT_CB4 4
+
+/* gradient-flow integrators (LettuceGauge/smearing/gradient_flow.hpp:31-56,120-168) */
+#define LT_FLOW_EULER 1
+#define LT_FLOW_RK3   3
+
+const char *lt_last_error(void);
+int lt_version(int *major, int *minor);
+
+/* ---- context ------------------------------------------------------------- */
+int lt_ctx_create(int device, lt_ctx *out);
+int lt_ctx_destroy(lt_ctx ctx);
+int lt_ctx_synchronize(lt_ctx ctx);
+/* the context's HIP stream (a hipStream_t), for callers that time or order work */
+int lt_ctx_stream(lt_ctx ctx, void **stream);
+/* per-kernel timing with HIP events recorded around every launch on the context
+ * stream; query returns launches and total milliseconds since enable/reset */
+int lt_ctx_profile(lt_ctx ctx, int enable);
+int lt_ctx_profile_query(lt_ctx ctx, const char *kernel, int *launches, double *total_ms);
+
+/* ---- gauge fields ---------------------------------------------------------
+ * GaugeField4D<Nt,Nx,Ny,Nz,Matrix_SU3> (lattice.hpp:87-394) with runtime
+ * extents; all extents must be even (Checkerboard, iterators.hpp:68).          */
+int lt_field_create(lt_ctx ctx, int nt, int nx, int ny, int nz, lt_field *out);
+int lt_field_destroy(lt_field f);
+int lt_field_shape(lt_field f, int shape[4]);
+/* host AoS <-> device (GaugeField4D::operator() element access, lattice.hpp:140-168) */
+int lt_field_upload(lt_field f, const double *aos);
+int lt_field_download(lt_field f, double *aos);
+/* copy assignment and pointer swap (lattice.hpp:110-127, 238-246) */
+int lt_field_copy(lt_field dst, lt_field src);
+int lt_field_swap(lt_field a, lt_field b);
+/* SetToIdentity (lattice.hpp:170-183) */
+int lt_field_set_identity(lt_field f);
+/* Haar-random SU(3) links from Philox key = seed (no reference counterpart;
+ * the reference starts from identity or loads a configuration) */
+int lt_field_set_haar(lt_field f, uint64_t seed);
+/* GramSchmidt on every link (su3.hpp:370-394) */
+int lt_field_reunitarize(lt_field f);
+/* max |U^+U - 1| and number of non-finite entries (SU3::Tests, su3.hpp:465-659) */
+int lt_field_check(lt_field f, double *max_unitarity_dev, long long *n_nonfinite);
+/* counter-based RNG state of the field (replaces PRNG4D, random/prng_wrapper.hpp:43-379) */
+int lt_field_set_rng(lt_field f, uint64_t seed, uint64_t counter);
+int lt_field_get_rng(lt_field f, uint64_t *seed, uint64_t *counter);
+
+/* ---- actions and local updates ------------------------------------------- */
+/* Rectangular<1>::Staple / WilsonAction::Staple for every link into out
+ * (rectangular_action.hpp:103-184, wilson_action.hpp:174-244) */
+int lt_staple_field(lt_field u, lt_field out);
+/* n sweeps of OverrelaxationSubgroupKernel driven by Iterator::Checkerboard[4]
+ * (overrelaxation.hpp:68-117, iterators.hpp:65-169) */
+int lt_or_sweep(lt_field u, int n_sweep, int schedule);
+/* n sweeps of HeatbathKernel (heatbath.hpp:30-130) at coupling beta; draws come
+ * from the field's Philox state, whose counter advances by one per sweep */
+int lt_heatbath_sweep(lt_field u, double beta, int n_sweep, int schedule);
+
+/* ---- observables ---------------------------------------------------------- */
+/* PlaquetteSum (plaquette.hpp:26-43) */
+int lt_plaquette_sum(lt_field u, double *out);
+/* WilsonAction::Action / Rectangular<1>::Action (wilson_action.hpp:23-41) */
+int lt_wilson_action(lt_field u, double beta, double *out);
+/* TopChargeClover (topological_charge.hpp:97-231); q_t (Nt entries, may be NULL)
+ * as TopChargeCloverTimeslice (:337-364) */
+int lt_topcharge_clover(lt_field u, double *Q, double *q_t);
+/* EnergyDensity::Clover / CloverTimeslice on traceless F (field_strength_tensor.hpp:
+ * 107-134,154-195) and EnergyDensity::Plaquette (:140-145); e_t may be NULL */
+int lt_energy(lt_field u, double *e_plaq, double *e_clov, double *e_t);
+/* all of the above in one fused pass: out[0]=plaquette sum, out[1]=Q, out[2]=E_plaq,
+ * out[3]=E_clov, then q_t[Nt], e_t[Nt]; out must hold 4 + 2*Nt doubles */
+int lt_observables(lt_field u, double *out);
+
+/* ---- smearing -------------------------------------------------------------- */
+/* StoutSmearing4D applied n times (stout_smearing.hpp:29-57, StoutSmearingN :61-75);
+ * in and out may be the same field */
+int lt_stout(lt_field in, lt_field out, double rho, int n_levels);
+/* GradientFlowKernel<Integrator, Rectangular<1>> (gradient_flow.hpp:252-367):
+ * create binds (unflowed, flowed) like the reference functor; run copies
+ * unflowed->flowed then integrates n steps, resume continues without copying */
+int lt_flow_create(lt_field unflowed, lt_field flowed, int integrator, double eps, lt_flow *out);
+int lt_flow_destroy(lt_flow fl);
+int lt_flow_run(lt_flow fl, int n_steps);
+int lt_flow_resume(lt_flow fl, int n_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,290 @@
+"""lettuce_amd -- MI355X-native gauge-field hot path of Lettuce.
+
+A thin Python mirror of the reference's call surface over the C ABI of
+liblettuce_hip.so (include/lettuce_hip.h).  Names follow the reference:
+
+  reference (LettuceGauge/...)                       here
+  GaugeField4D (lattice.hpp:87-394)                  GaugeField
+  Iterator::Checkerboard / Checkerboard4             Iterator.Checkerboard / .Checkerboard4
+  HeatbathKernel (updates/heatbath.hpp:30-130)       HeatbathKernel
+  OverrelaxationSubgroupKernel (overrelaxation.hpp)  OverrelaxationSubgroupKernel
+  PlaquetteSum (observables/plaquette.hpp:26-43)     PlaquetteSum
+  TopChargeClover (topological_charge.hpp:97-231)    TopChargeClover, TopChargeCloverTimeslice
+  EnergyDensity (field_strength_tensor.hpp:136-196)  EnergyDensity.Plaquette / .Clover / .CloverTimeslice
+  WilsonAction::Action (wilson_action.hpp:23-41)     WilsonAction.Action
+  StoutSmearing4D / StoutSmearingN (stout_smearing)  StoutSmearing4D, StoutSmearingN
+  GradientFlowKernel (smearing/gradient_flow.hpp)    GradientFlowKernel
+
+Every call runs HIP kernels on the GPU; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import LettuceError, call
+
+__all__ = [
+    "Context", "GaugeField", "Iterator", "HeatbathKernel", "OverrelaxationSubgroupKernel",
+    "PlaquetteSum", "TopChargeClover", "TopChargeCloverTimeslice", "EnergyDensity", "WilsonAction",
+    "StoutSmearing4D", "StoutSmearingN", "GradientFlowKernel", "Observables", "LettuceError",
+    "CB2", "CB4",
+]
+
+CB2 = 2
+CB4 = 4
+FLOW_EULER = 1
+FLOW_RK3 = 3
+
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class Context:
+    """One device and one HIP stream (lt_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        call("lt_ctx_create", device, ctypes.byref(h))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        call("lt_ctx_synchronize", self._h)
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        call("lt_ctx_stream", self._h, ctypes.byref(s))
+        return s.value
+
+    def profile(self, enable=True):
+        call("lt_ctx_profile", self._h, 1 if enable else 0)
+
+    def kernel_time(self, name):
+        n = ctypes.c_int()
+        ms = ctypes.c_double()
+        call("lt_ctx_profile_query", self._h, name.encode(), ctypes.byref(n), ctypes.byref(ms))
+        return n.value, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            call("lt_ctx_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GaugeField:
+    """Device gauge field with runtime extents (Nt, Nx, Ny, Nz), all even."""
+
+    def __init__(self, ctx, shape):
+        self.ctx = ctx
+        self.shape = tuple(int(x) for x in shape)
+        h = ctypes.c_void_p()
+        call("lt_field_create", ctx.handle, *self.shape, ctypes.byref(h))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def volume(self):
+        return int(np.prod(self.shape))
+
+    def upload(self, aos):
+        a = np.ascontiguousarray(aos, dtype=np.float64)
+        if a.size != self.volume * 72:
+            raise ValueError("upload: array size does not match the lattice")
+        call("lt_field_upload", self._h, _dp(a))
+
+    def download(self):
+        a = np.empty((self.volume, 4, 18), dtype=np.float64)
+        call("lt_field_download", self._h, _dp(a))
+        return a
+
+    def SetToIdentity(self):
+        call("lt_field_set_identity", self._h)
+
+    def SetToHaar(self, seed):
+        call("lt_field_set_haar", self._h, seed)
+
+    def reunitarize(self):
+        call("lt_field_reunitarize", self._h)
+
+    def assign(self, other):
+        """Copy assignment U = other (lattice.hpp:110-127)."""
+        call("lt_field_copy", self._h, other.handle)
+
+    def check(self):
+        dev = ctypes.c_double()
+        bad = ctypes.c_longlong()
+        call("lt_field_check", self._h, ctypes.byref(dev), ctypes.byref(bad))
+        return dev.value, bad.value
+
+    def set_rng(self, seed, counter=0):
+        call("lt_field_set_rng", self._h, seed, counter)
+
+    def get_rng(self):
+        s = ctypes.c_uint64()
+        c = ctypes.c_uint64()
+        call("lt_field_get_rng", self._h, ctypes.byref(s), ctypes.byref(c))
+        return s.value, c.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            call("lt_field_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def Swap(a, b):
+    """Pointer swap of two fields (lattice.hpp:238-246)."""
+    call("lt_field_swap", a.handle, b.handle)
+
+
+def Staple(U, out):
+    """Staple of every link into out (rectangular_action.hpp:103-184)."""
+    call("lt_staple_field", U.handle, out.handle)
+
+
+class HeatbathKernel:
+    """HeatbathKernel(U, Action, prng) (heatbath.hpp:106-108); beta plays the Action/global role,
+    the field's Philox state plays the PRNG role."""
+
+    def __init__(self, U, beta):
+        self.U = U
+        self.beta = float(beta)
+
+    def sweep(self, n, schedule):
+        call("lt_heatbath_sweep", self.U.handle, self.beta, n, schedule)
+
+
+class OverrelaxationSubgroupKernel:
+    """OverrelaxationSubgroupKernel(U, Action) (overrelaxation.hpp:68-117)."""
+
+    def __init__(self, U, beta=None):
+        self.U = U
+
+    def sweep(self, n, schedule):
+        call("lt_or_sweep", self.U.handle, n, schedule)
+
+
+class Iterator:
+    """Iterator::Checkerboard / Checkerboard4 (iterators/iterators.hpp:65-169): whole sweeps."""
+
+    @staticmethod
+    def Checkerboard(kernel, n_sweep=1):
+        kernel.sweep(n_sweep, CB2)
+
+    @staticmethod
+    def Checkerboard4(kernel, n_sweep=1):
+        kernel.sweep(n_sweep, CB4)
+
+
+def Observables(U):
+    """Fused plaquette / clover-Q / energy pass; returns a dict."""
+    nt = U.shape[0]
+    out = np.zeros(4 + 2 * nt)
+    call("lt_observables", U.handle, _dp(out))
+    return {"plaquette_sum": out[0], "Q": out[1], "E_plaq": out[2], "E_clov": out[3],
+            "Q_t": out[4:4 + nt].copy(), "E_t": out[4 + nt:].copy()}
+
+
+def PlaquetteSum(U):
+    v = ctypes.c_double()
+    call("lt_plaquette_sum", U.handle, ctypes.byref(v))
+    return v.value
+
+
+def TopChargeClover(U):
+    v = ctypes.c_double()
+    call("lt_topcharge_clover", U.handle, ctypes.byref(v), None)
+    return v.value
+
+
+def TopChargeCloverTimeslice(U):
+    q = ctypes.c_double()
+    qt = np.zeros(U.shape[0])
+    call("lt_topcharge_clover", U.handle, ctypes.byref(q), _dp(qt))
+    return qt
+
+
+class EnergyDensity:
+    @staticmethod
+    def Plaquette(U):
+        e = ctypes.c_double()
+        call("lt_energy", U.handle, ctypes.byref(e), None, None)
+        return e.value
+
+    @staticmethod
+    def Clover(U):
+        e = ctypes.c_double()
+        call("lt_energy", U.handle, None, ctypes.byref(e), None)
+        return e.value
+
+    @staticmethod
+    def CloverTimeslice(U):
+        et = np.zeros(U.shape[0])
+        call("lt_energy", U.handle, None, None, _dp(et))
+        return et
+
+
+class WilsonAction:
+    @staticmethod
+    def Action(U, beta):
+        v = ctypes.c_double()
+        call("lt_wilson_action", U.handle, beta, ctypes.byref(v))
+        return v.value
+
+
+def StoutSmearing4D(U_in, U_out, rho=0.12):
+    call("lt_stout", U_in.handle, U_out.handle, rho, 1)
+
+
+def StoutSmearingN(U_in, U_out, n, rho=0.12):
+    call("lt_stout", U_in.handle, U_out.handle, rho, n)
+
+
+class GradientFlowKernel:
+    """GradientFlowKernel(U_unflowed, U_flowed, Integrator, Action, epsilon) (gradient_flow.hpp:252-367)."""
+
+    def __init__(self, U_unflowed, U_flowed, integrator="rk3", epsilon=0.01):
+        code = {"rk3": FLOW_RK3, "euler": FLOW_EULER}[integrator.lower()]
+        h = ctypes.c_void_p()
+        call("lt_flow_create", U_unflowed.handle, U_flowed.handle, code, epsilon, ctypes.byref(h))
+        self._h = h
+        self.U_unflowed = U_unflowed
+        self.U_flowed = U_flowed
+
+    def __call__(self, n_step):
+        call("lt_flow_run", self._h, n_step)
+
+    def Resume(self, n_step):
+        call("lt_flow_resume", self._h, n_step)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            call("lt_flow_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,95 @@
+"""ctypes declarations of the C ABI in include/lettuce_hip.h.
+
+Loads lettuce_amd/liblettuce_hip.so and nothing else: there is no CPU fallback.
+If the library is missing or fails to load, importing raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblettuce_hip.so")
+
+c_ctx = ctypes.c_void_p
+c_field = ctypes.c_void_p
+c_flow = ctypes.c_void_p
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.c_int
+
+# name -> (argtypes); every entry point returns int status
+SIGNATURES = {
+    "lt_version": [ctypes.POINTER(I), ctypes.POINTER(I)],
+    "lt_ctx_create": [I, ctypes.POINTER(c_ctx)],
+    "lt_ctx_destroy": [c_ctx],
+    "lt_ctx_synchronize": [c_ctx],
+    "lt_ctx_stream": [c_ctx, ctypes.POINTER(ctypes.c_void_p)],
+    "lt_ctx_profile": [c_ctx, I],
+    "lt_ctx_profile_query": [c_ctx, ctypes.c_char_p, ctypes.POINTER(I), D],
+    "lt_field_create": [c_ctx, I, I, I, I, ctypes.POINTER(c_field)],
+    "lt_field_destroy": [c_field],
+    "lt_field_shape": [c_field, ctypes.POINTER(I)],
+    "lt_field_upload": [c_field, D],
+    "lt_field_download": [c_field, D],
+    "lt_field_copy": [c_field, c_field],
+    "lt_field_swap": [c_field, c_field],
+    "lt_field_set_identity": [c_field],
+    "lt_field_set_haar": [c_field, ctypes.c_uint64],
+    "lt_field_reunitarize": [c_field],
+    "lt_field_check": [c_field, D, ctypes.POINTER(ctypes.c_longlong)],
+    "lt_field_set_rng": [c_field, ctypes.c_uint64, ctypes.c_uint64],
+    "lt_field_get_rng": [c_field, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
+    "lt_staple_field": [c_field, c_field],
+    "lt_or_sweep": [c_field, I, I],
+    "lt_heatbath_sweep": [c_field, ctypes.c_double, I, I],
+    "lt_plaquette_sum": [c_field, D],
+    "lt_wilson_action": [c_field, ctypes.c_double, D],
+    "lt_topcharge_clover": [c_field, D, D],
+    "lt_energy": [c_field, D, D, D],
+    "lt_observables": [c_field, D],
+    "lt_stout": [c_field, c_field, ctypes.c_double, I],
+    "lt_flow_create": [c_field, c_field, I, ctypes.c_double, ctypes.POINTER(c_flow)],
+    "lt_flow_destroy": [c_flow],
+    "lt_flow_run": [c_flow, I],
+    "lt_flow_resume": [c_flow, I],
+}
+
+
+class LettuceError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed ({code}): {msg}")
+        self.code = code
+
+
+def load(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `python -m lettuce_amd.build` (hipcc, gfx950); "
+            "there is no CPU fallback")
+    lib = ctypes.CDLL(path)
+    lib.lt_last_error.restype = ctypes.c_char_p
+    lib.lt_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = ctypes.c_int
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        raise LettuceError(name, rc, lib().lt_last_error().decode(errors="replace"))
+    return rc
+
+
+def exported_symbols():
+    return ["lt_last_error"] + list(SIGNATURES)

@@ -1,0 +1,604 @@
+// lt_api.cpp -- the C ABI (include/lettuce_hip.h): argument checking, object
+// lifetimes, host<->device transfers and the host-side sequencing of the hot
+// path (sweep schedules, smearing levels, flow integrators).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "lt_internal.hpp"
+
+namespace lt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+Geom make_geom(int nt, int nx, int ny, int nz)
+{
+    Geom g{};
+    g.L[0] = nt;
+    g.L[1] = nx;
+    g.L[2] = ny;
+    g.L[3] = nz;
+    g.st[3] = 1;
+    g.st[2] = nz;
+    g.st[1] = ny * nz;
+    g.st[0] = nx * ny * nz;
+    g.V = nt * nx * ny * nz;
+    g.Vh = g.V / 2;
+    return g;
+}
+
+}  // namespace lt
+
+double* lt_ctx_s::scratch_for(size_t n)
+{
+    if (n > scratch_doubles) {
+        if (scratch) LT_HIP_CHECK(hipFree(scratch));
+        scratch = nullptr;
+        scratch_doubles = 0;
+        if (hipMalloc(&scratch, n * sizeof(double)) != hipSuccess)
+            throw lt::LtError{LT_ENOMEM, "scratch allocation failed"};
+        scratch_doubles = n;
+    }
+    return scratch;
+}
+
+double* lt_ctx_s::host_for(size_t n)
+{
+    if (n > host_doubles) {
+        if (host) LT_HIP_CHECK(hipHostFree(host));
+        host = nullptr;
+        host_doubles = 0;
+        if (hipHostMalloc((void**)&host, n * sizeof(double), hipHostMallocDefault) != hipSuccess)
+            throw lt::LtError{LT_ENOMEM, "pinned host allocation failed"};
+        host_doubles = n;
+    }
+    return host;
+}
+
+void lt_ctx_s::collect_profile()
+{
+    if (pending.empty()) return;
+    LT_HIP_CHECK(hipStreamSynchronize(stream));
+    for (auto& p : pending) {
+        float ms = 0.0f;
+        LT_HIP_CHECK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        auto& e = prof[p.first];
+        e.launches += 1;
+        e.ms += ms;
+        event_pool.push_back(p.second);
+    }
+    pending.clear();
+}
+
+namespace {
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const lt::LtError& e) {
+        lt::set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        lt::set_error(e.what());
+        return LT_ESTATE;
+    }
+}
+
+int fail(int code, const std::string& msg)
+{
+    lt::set_error(msg);
+    return code;
+}
+
+void use_device(lt_ctx ctx) { LT_HIP_CHECK(hipSetDevice(ctx->device)); }
+
+bool same_shape(lt_field a, lt_field b)
+{
+    for (int i = 0; i < 4; ++i)
+        if (a->g.L[i] != b->g.L[i]) return false;
+    return true;
+}
+
+double2* alloc_links(const lt::Geom& g)
+{
+    double2* p = nullptr;
+    if (hipMalloc(&p, (size_t)72 * g.Vh * sizeof(double2)) != hipSuccess)
+        throw lt::LtError{LT_ENOMEM, "link buffer allocation failed"};
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lt_last_error(void) { return lt::g_last_error.c_str(); }
+
+int lt_version(int* major, int* minor)
+{
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return LT_OK;
+}
+
+int lt_ctx_create(int device, lt_ctx* out)
+{
+    return guarded([&] {
+        if (!out) return fail(LT_EINVAL, "lt_ctx_create: null out");
+        int n = 0;
+        LT_HIP_CHECK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) return fail(LT_EINVAL, "lt_ctx_create: no such device");
+        LT_HIP_CHECK(hipSetDevice(device));
+        auto* c = new lt_ctx_s();
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(LT_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+        *out = c;
+        return LT_OK;
+    });
+}
+
+int lt_ctx_destroy(lt_ctx ctx)
+{
+    if (!ctx) return LT_OK;
+    return guarded([&] {
+        use_device(ctx);
+        LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (auto& p : ctx->pending) ctx->event_pool.push_back(p.second);
+        for (auto& p : ctx->event_pool) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        if (ctx->host) (void)hipHostFree(ctx->host);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return LT_OK;
+    });
+}
+
+int lt_ctx_synchronize(lt_ctx ctx)
+{
+    if (!ctx) return fail(LT_EINVAL, "null context");
+    return guarded([&] {
+        LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return LT_OK;
+    });
+}
+
+int lt_ctx_stream(lt_ctx ctx, void** stream)
+{
+    if (!ctx || !stream) return fail(LT_EINVAL, "null argument");
+    *stream = (void*)ctx->stream;
+    return LT_OK;
+}
+
+int lt_ctx_profile(lt_ctx ctx, int enable)
+{
+    if (!ctx) return fail(LT_EINVAL, "null context");
+    return guarded([&] {
+        ctx->collect_profile();
+        ctx->prof.clear();
+        ctx->profiling = enable != 0;
+        return LT_OK;
+    });
+}
+
+int lt_ctx_profile_query(lt_ctx ctx, const char* kernel, int* launches, double* total_ms)
+{
+    if (!ctx || !kernel) return fail(LT_EINVAL, "null argument");
+    return guarded([&] {
+        ctx->collect_profile();
+        auto it = ctx->prof.find(kernel);
+        if (launches) *launches = it == ctx->prof.end() ? 0 : it->second.launches;
+        if (total_ms) *total_ms = it == ctx->prof.end() ? 0.0 : it->second.ms;
+        return LT_OK;
+    });
+}
+
+int lt_field_create(lt_ctx ctx, int nt, int nx, int ny, int nz, lt_field* out)
+{
+    if (!ctx || !out) return fail(LT_EINVAL, "lt_field_create: null argument");
+    if (nt < 2 || nx < 2 || ny < 2 || nz < 2 || (nt | nx | ny | nz) & 1)
+        return fail(LT_EINVAL, "lt_field_create: extents must be even and >= 2");
+    if ((double)nt * nx * ny * nz > 1.0e9) return fail(LT_EINVAL, "lt_field_create: volume too large");
+    return guarded([&] {
+        use_device(ctx);
+        auto* f = new lt_field_s();
+        f->ctx = ctx;
+        f->g = lt::make_geom(nt, nx, ny, nz);
+        try {
+            f->d = alloc_links(f->g);
+        } catch (...) {
+            delete f;
+            throw;
+        }
+        *out = f;
+        return LT_OK;
+    });
+}
+
+int lt_field_destroy(lt_field f)
+{
+    if (!f) return LT_OK;
+    return guarded([&] {
+        use_device(f->ctx);
+        LT_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+        if (f->d) LT_HIP_CHECK(hipFree(f->d));
+        delete f;
+        return LT_OK;
+    });
+}
+
+int lt_field_shape(lt_field f, int shape[4])
+{
+    if (!f || !shape) return fail(LT_EINVAL, "null argument");
+    for (int i = 0; i < 4; ++i) shape[i] = f->g.L[i];
+    return LT_OK;
+}
+
+int lt_field_upload(lt_field f, const double* aos)
+{
+    if (!f || !aos) return fail(LT_EINVAL, "lt_field_upload: null argument");
+    return guarded([&] {
+        use_device(f->ctx);
+        size_t bytes = (size_t)4 * f->g.V * 18 * sizeof(double);
+        double* stage = nullptr;
+        if (hipMalloc(&stage, bytes) != hipSuccess) throw lt::LtError{LT_ENOMEM, "upload staging allocation failed"};
+        hipError_t e = hipMemcpyAsync(stage, aos, bytes, hipMemcpyHostToDevice, f->ctx->stream);
+        if (e == hipSuccess) {
+            try {
+                lt::k_pack(f, stage);
+            } catch (...) {
+                (void)hipStreamSynchronize(f->ctx->stream);
+                (void)hipFree(stage);
+                throw;
+            }
+            e = hipStreamSynchronize(f->ctx->stream);
+        }
+        (void)hipFree(stage);
+        if (e != hipSuccess) throw lt::LtError{LT_EHIP, std::string("upload: ") + hipGetErrorString(e)};
+        return LT_OK;
+    });
+}
+
+int lt_field_download(lt_field f, double* aos)
+{
+    if (!f || !aos) return fail(LT_EINVAL, "lt_field_download: null argument");
+    return guarded([&] {
+        use_device(f->ctx);
+        size_t bytes = (size_t)4 * f->g.V * 18 * sizeof(double);
+        double* stage = nullptr;
+        if (hipMalloc(&stage, bytes) != hipSuccess) throw lt::LtError{LT_ENOMEM, "download staging allocation failed"};
+        hipError_t e = hipSuccess;
+        try {
+            lt::k_unpack(f, stage);
+            e = hipMemcpyAsync(aos, stage, bytes, hipMemcpyDeviceToHost, f->ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
+        } catch (...) {
+            (void)hipStreamSynchronize(f->ctx->stream);
+            (void)hipFree(stage);
+            throw;
+        }
+        (void)hipFree(stage);
+        if (e != hipSuccess) throw lt::LtError{LT_EHIP, std::string("download: ") + hipGetErrorString(e)};
+        return LT_OK;
+    });
+}
+
+int lt_field_copy(lt_field dst, lt_field src)
+{
+    if (!dst || !src) return fail(LT_EINVAL, "lt_field_copy: null argument");
+    if (!same_shape(dst, src)) return fail(LT_ESTATE, "lt_field_copy: shape mismatch");
+    if (dst == src) return LT_OK;
+    return guarded([&] {
+        use_device(dst->ctx);
+        LT_HIP_CHECK(hipMemcpyAsync(dst->d, src->d, dst->bytes(), hipMemcpyDeviceToDevice, dst->ctx->stream));
+        return LT_OK;
+    });
+}
+
+int lt_field_swap(lt_field a, lt_field b)
+{
+    if (!a || !b) return fail(LT_EINVAL, "lt_field_swap: null argument");
+    if (!same_shape(a, b) || a->ctx != b->ctx) return fail(LT_ESTATE, "lt_field_swap: shape/context mismatch");
+    std::swap(a->d, b->d);
+    return LT_OK;
+}
+
+int lt_field_set_identity(lt_field f)
+{
+    if (!f) return fail(LT_EINVAL, "null field");
+    return guarded([&] {
+        use_device(f->ctx);
+        lt::k_identity(f);
+        return LT_OK;
+    });
+}
+
+int lt_field_set_haar(lt_field f, uint64_t seed)
+{
+    if (!f) return fail(LT_EINVAL, "null field");
+    return guarded([&] {
+        use_device(f->ctx);
+        lt::k_haar(f, seed);
+        return LT_OK;
+    });
+}
+
+int lt_field_reunitarize(lt_field f)
+{
+    if (!f) return fail(LT_EINVAL, "null field");
+    return guarded([&] {
+        use_device(f->ctx);
+        lt::k_reunitarize(f);
+        return LT_OK;
+    });
+}
+
+int lt_field_check(lt_field f, double* max_dev, long long* n_nonfinite)
+{
+    if (!f || !max_dev || !n_nonfinite) return fail(LT_EINVAL, "null argument");
+    return guarded([&] {
+        use_device(f->ctx);
+        lt::k_check(f, max_dev, n_nonfinite);
+        return LT_OK;
+    });
+}
+
+int lt_field_set_rng(lt_field f, uint64_t seed, uint64_t counter)
+{
+    if (!f) return fail(LT_EINVAL, "null field");
+    f->seed = seed;
+    f->counter = counter;
+    return LT_OK;
+}
+
+int lt_field_get_rng(lt_field f, uint64_t* seed, uint64_t* counter)
+{
+    if (!f) return fail(LT_EINVAL, "null field");
+    if (seed) *seed = f->seed;
+    if (counter) *counter = f->counter;
+    return LT_OK;
+}
+
+int lt_staple_field(lt_field u, lt_field out)
+{
+    if (!u || !out) return fail(LT_EINVAL, "null field");
+    if (!same_shape(u, out) || u == out) return fail(LT_ESTATE, "lt_staple_field: need a distinct field of equal shape");
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_staple_field(u, out);
+        return LT_OK;
+    });
+}
+
+static int check_schedule(lt_field u, int schedule)
+{
+    if (schedule != LT_CB2 && schedule != LT_CB4) return fail(LT_EINVAL, "schedule must be LT_CB2 or LT_CB4");
+    if (schedule == LT_CB4)
+        for (int i = 0; i < 4; ++i)
+            if (u->g.L[i] % 4) return fail(LT_EINVAL, "LT_CB4 needs extents divisible by 4");
+    return LT_OK;
+}
+
+int lt_or_sweep(lt_field u, int n_sweep, int schedule)
+{
+    if (!u || n_sweep < 0) return fail(LT_EINVAL, "lt_or_sweep: bad argument");
+    int rc = check_schedule(u, schedule);
+    if (rc) return rc;
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_or_sweep(u, n_sweep, schedule);
+        return LT_OK;
+    });
+}
+
+int lt_heatbath_sweep(lt_field u, double beta, int n_sweep, int schedule)
+{
+    if (!u || n_sweep < 0 || !(beta > 0.0)) return fail(LT_EINVAL, "lt_heatbath_sweep: bad argument");
+    int rc = check_schedule(u, schedule);
+    if (rc) return rc;
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_hb_sweep(u, beta, n_sweep, schedule);
+        return LT_OK;
+    });
+}
+
+// out: [plaq_sum, Q, E_plaq, E_clov, q_t[Nt], e_t[Nt]]
+int lt_observables(lt_field u, double* out)
+{
+    if (!u || !out) return fail(LT_EINVAL, "null argument");
+    return guarded([&] {
+        use_device(u->ctx);
+        const int nt = u->g.L[0];
+        double* h = u->ctx->host_for(3 * (size_t)nt);
+        lt::k_observables(u, h);
+        const double V = (double)u->g.V;
+        const double qnorm = -1.0 / (256.0 * M_PI * M_PI);
+        double p = 0.0, q = 0.0, e = 0.0;
+        for (int t = 0; t < nt; ++t) {
+            double qt = qnorm * h[3 * t + 1];
+            double et = h[3 * t + 2] / V;
+            out[4 + t] = qt;
+            out[4 + nt + t] = et;
+            p += h[3 * t];
+            q += qt;
+            e += et;
+        }
+        out[0] = p;
+        out[1] = q;
+        out[2] = 2.0 * (18.0 - p / V);
+        out[3] = e;
+        return LT_OK;
+    });
+}
+
+int lt_plaquette_sum(lt_field u, double* out)
+{
+    if (!u || !out) return fail(LT_EINVAL, "null argument");
+    std::vector<double> o(4 + 2 * (size_t)u->g.L[0]);
+    int rc = lt_observables(u, o.data());
+    if (rc == LT_OK) *out = o[0];
+    return rc;
+}
+
+int lt_wilson_action(lt_field u, double beta, double* out)
+{
+    if (!u || !out) return fail(LT_EINVAL, "null argument");
+    double p = 0.0;
+    int rc = lt_plaquette_sum(u, &p);
+    if (rc == LT_OK) *out = beta * (6.0 * (double)u->g.V - p / 3.0);
+    return rc;
+}
+
+int lt_topcharge_clover(lt_field u, double* Q, double* q_t)
+{
+    if (!u || !Q) return fail(LT_EINVAL, "null argument");
+    const int nt = u->g.L[0];
+    std::vector<double> o(4 + 2 * (size_t)nt);
+    int rc = lt_observables(u, o.data());
+    if (rc) return rc;
+    *Q = o[1];
+    if (q_t) std::memcpy(q_t, o.data() + 4, nt * sizeof(double));
+    return LT_OK;
+}
+
+int lt_energy(lt_field u, double* e_plaq, double* e_clov, double* e_t)
+{
+    if (!u) return fail(LT_EINVAL, "null argument");
+    const int nt = u->g.L[0];
+    std::vector<double> o(4 + 2 * (size_t)nt);
+    int rc = lt_observables(u, o.data());
+    if (rc) return rc;
+    if (e_plaq) *e_plaq = o[2];
+    if (e_clov) *e_clov = o[3];
+    if (e_t) std::memcpy(e_t, o.data() + 4 + nt, nt * sizeof(double));
+    return LT_OK;
+}
+
+int lt_stout(lt_field in, lt_field out, double rho, int n_levels)
+{
+    if (!in || !out || n_levels < 0) return fail(LT_EINVAL, "lt_stout: bad argument");
+    if (!same_shape(in, out) || in->ctx != out->ctx) return fail(LT_ESTATE, "lt_stout: shape/context mismatch");
+    return guarded([&] {
+        use_device(in->ctx);
+        lt_ctx ctx = in->ctx;
+        if (n_levels == 0) {
+            if (in != out)
+                LT_HIP_CHECK(hipMemcpyAsync(out->d, in->d, in->bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+            return LT_OK;
+        }
+        // ping-pong so that the last level lands in out->d
+        double2* tmp = alloc_links(in->g);
+        const double2* src = in->d;
+        for (int k = 0; k < n_levels; ++k) {
+            const bool last = k == n_levels - 1;
+            double2* dst;
+            if (last) dst = (src == out->d) ? tmp : out->d;
+            else dst = (src == tmp) ? out->d : tmp;
+            if (dst == src) dst = (dst == tmp) ? out->d : tmp;
+            lt::k_stout(ctx, in->g, src, dst, rho);
+            src = dst;
+        }
+        if (src != out->d)
+            LT_HIP_CHECK(hipMemcpyAsync(out->d, src, in->bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+        LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        LT_HIP_CHECK(hipFree(tmp));
+        return LT_OK;
+    });
+}
+
+int lt_flow_create(lt_field unflowed, lt_field flowed, int integrator, double eps, lt_flow* out)
+{
+    if (!unflowed || !flowed || !out) return fail(LT_EINVAL, "lt_flow_create: null argument");
+    if (integrator != LT_FLOW_EULER && integrator != LT_FLOW_RK3) return fail(LT_EINVAL, "unknown integrator");
+    if (!same_shape(unflowed, flowed) || unflowed->ctx != flowed->ctx || unflowed == flowed)
+        return fail(LT_ESTATE, "lt_flow_create: need two distinct fields of equal shape");
+    return guarded([&] {
+        use_device(flowed->ctx);
+        auto* fl = new lt_flow_s();
+        fl->unflowed = unflowed;
+        fl->flowed = flowed;
+        fl->integrator = integrator;
+        fl->eps = eps;
+        try {
+            fl->z = alloc_links(flowed->g);
+            fl->tmp = alloc_links(flowed->g);
+        } catch (...) {
+            if (fl->z) (void)hipFree(fl->z);
+            delete fl;
+            throw;
+        }
+        *out = fl;
+        return LT_OK;
+    });
+}
+
+int lt_flow_destroy(lt_flow fl)
+{
+    if (!fl) return LT_OK;
+    return guarded([&] {
+        use_device(fl->flowed->ctx);
+        LT_HIP_CHECK(hipStreamSynchronize(fl->flowed->ctx->stream));
+        (void)hipFree(fl->z);
+        (void)hipFree(fl->tmp);
+        delete fl;
+        return LT_OK;
+    });
+}
+
+// One integrator step on fl->flowed; each stage writes the other buffer and the
+// buffers are exchanged, so the flowed field always owns the newest links.
+static void flow_steps(lt_flow fl, int n)
+{
+    lt_field u = fl->flowed;
+    lt_ctx ctx = u->ctx;
+    const double e = fl->eps;
+    auto stage = [&](double c_old, double c_new, double c_exp, int read_z, int write_z) {
+        lt::k_flow_stage(ctx, u->g, u->d, fl->tmp, fl->z, c_old, c_new, c_exp, read_z, write_z);
+        std::swap(u->d, fl->tmp);
+    };
+    for (int k = 0; k < n; ++k) {
+        if (fl->integrator == LT_FLOW_EULER) {
+            stage(0.0, e, 1.0, 0, 0);
+        } else {
+            stage(0.0, e, 0.25, 0, 1);
+            stage(-17.0 / 36.0, 8.0 / 9.0 * e, 1.0, 1, 1);
+            stage(-1.0, 3.0 / 4.0 * e, 1.0, 1, 0);
+        }
+    }
+}
+
+int lt_flow_run(lt_flow fl, int n_steps)
+{
+    if (!fl || n_steps < 0) return fail(LT_EINVAL, "lt_flow_run: bad argument");
+    return guarded([&] {
+        use_device(fl->flowed->ctx);
+        LT_HIP_CHECK(hipMemcpyAsync(fl->flowed->d, fl->unflowed->d, fl->flowed->bytes(), hipMemcpyDeviceToDevice,
+                                    fl->flowed->ctx->stream));
+        flow_steps(fl, n_steps);
+        return LT_OK;
+    });
+}
+
+int lt_flow_resume(lt_flow fl, int n_steps)
+{
+    if (!fl || n_steps < 0) return fail(LT_EINVAL, "lt_flow_resume: bad argument");
+    return guarded([&] {
+        use_device(fl->flowed->ctx);
+        flow_steps(fl, n_steps);
+        return LT_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,447 @@
+// lt_device.hpp -- device-side building blocks for the gfx950 gauge-field kernels:
+// FP64 complex and 3x3 matrix arithmetic, the even/odd structure-of-arrays link
+// layout, lattice neighbour arithmetic, the Philox4x32-10 counter RNG, SU(2)
+// subgroup helpers, Gram-Schmidt reunitarisation and the Cayley-Hamilton
+// exponential.  Everything here is FP64 VALU code; MFMA is not used because a
+// 3x3 complex product is not a dense contraction worth tiling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lt {
+
+// ---------------------------------------------------------------------------
+// complex double
+struct cd {
+    double re, im;
+};
+
+__device__ __forceinline__ cd mk(double r, double i) { return cd{r, i}; }
+__device__ __forceinline__ cd operator+(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cd operator-(cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cd operator-(cd a) { return cd{-a.re, -a.im}; }
+__device__ __forceinline__ cd operator*(cd a, cd b)
+{
+    return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cd operator*(double s, cd a) { return cd{s * a.re, s * a.im}; }
+__device__ __forceinline__ cd operator*(cd a, double s) { return cd{s * a.re, s * a.im}; }
+__device__ __forceinline__ cd conj(cd a) { return cd{a.re, -a.im}; }
+__device__ __forceinline__ cd mul_i(cd a) { return cd{-a.im, a.re}; }
+__device__ __forceinline__ cd& operator+=(cd& a, cd b) { a.re += b.re; a.im += b.im; return a; }
+__device__ __forceinline__ cd& operator-=(cd& a, cd b) { a.re -= b.re; a.im -= b.im; return a; }
+// a * conj(b)
+__device__ __forceinline__ cd mul_c(cd a, cd b)
+{
+    return cd{a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+// conj(a) * b
+__device__ __forceinline__ cd cmul(cd a, cd b)
+{
+    return cd{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 complex matrix, row-major
+struct M3 {
+    cd m[9];
+};
+
+__device__ __forceinline__ M3 m3_zero()
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.m[i] = cd{0.0, 0.0};
+    return r;
+}
+__device__ __forceinline__ M3 m3_id()
+{
+    M3 r = m3_zero();
+    r.m[0].re = r.m[4].re = r.m[8].re = 1.0;
+    return r;
+}
+// a * b
+__device__ __forceinline__ M3 mul(const M3& a, const M3& b)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            r.m[3 * i + j] = a.m[3 * i] * b.m[j] + a.m[3 * i + 1] * b.m[3 + j] + a.m[3 * i + 2] * b.m[6 + j];
+    return r;
+}
+// a * b^+
+__device__ __forceinline__ M3 mul_nd(const M3& a, const M3& b)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            r.m[3 * i + j] = mul_c(a.m[3 * i], b.m[3 * j]) + mul_c(a.m[3 * i + 1], b.m[3 * j + 1])
+                           + mul_c(a.m[3 * i + 2], b.m[3 * j + 2]);
+    return r;
+}
+// a^+ * b
+__device__ __forceinline__ M3 mul_dn(const M3& a, const M3& b)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            r.m[3 * i + j] = cmul(a.m[i], b.m[j]) + cmul(a.m[3 + i], b.m[3 + j]) + cmul(a.m[6 + i], b.m[6 + j]);
+    return r;
+}
+__device__ __forceinline__ M3 dag(const M3& a)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[3 * i + j] = conj(a.m[3 * j + i]);
+    return r;
+}
+__device__ __forceinline__ M3 add(const M3& a, const M3& b)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.m[i] = a.m[i] + b.m[i];
+    return r;
+}
+__device__ __forceinline__ M3 sub(const M3& a, const M3& b)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.m[i] = a.m[i] - b.m[i];
+    return r;
+}
+__device__ __forceinline__ void acc(M3& a, const M3& b)
+{
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.m[i] += b.m[i];
+}
+__device__ __forceinline__ M3 scale(double s, const M3& a)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.m[i] = s * a.m[i];
+    return r;
+}
+__device__ __forceinline__ M3 scale(cd s, const M3& a)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.m[i] = s * a.m[i];
+    return r;
+}
+__device__ __forceinline__ cd trace(const M3& a) { return a.m[0] + a.m[4] + a.m[8]; }
+// Re tr(a b)
+__device__ __forceinline__ double retr_mul(const M3& a, const M3& b)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s += a.m[3 * i + k].re * b.m[3 * k + i].re - a.m[3 * i + k].im * b.m[3 * k + i].im;
+    return s;
+}
+// traceless anti-hermitian part: 1/2 (A - A^+) - 1/6 tr(A - A^+)
+__device__ __forceinline__ M3 algebra(const M3& a)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[3 * i + j] = 0.5 * (a.m[3 * i + j] - conj(a.m[3 * j + i]));
+    // tr(A - A^+) = 2i * sum Im A_ii
+    double t = (2.0 * (a.m[0].im + a.m[4].im + a.m[8].im)) * (1.0 / 6.0);
+    r.m[0].im -= t;
+    r.m[4].im -= t;
+    r.m[8].im -= t;
+    return r;
+}
+// traceless hermitian part: 1/2 (A + A^+) - 1/6 tr(A + A^+)
+__device__ __forceinline__ M3 tl_herm(const M3& a)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[3 * i + j] = 0.5 * (a.m[3 * i + j] + conj(a.m[3 * j + i]));
+    double t = (2.0 * (a.m[0].re + a.m[4].re + a.m[8].re)) * (1.0 / 6.0);
+    r.m[0].re -= t;
+    r.m[4].re -= t;
+    r.m[8].re -= t;
+    return r;
+}
+
+// Normalise row 0, orthogonalise and normalise row 1, row 2 = conj(row0 x row1)
+__device__ __forceinline__ void gram_schmidt(M3& a)
+{
+    cd* m = a.m;
+    double n0 = 1.0 / sqrt(m[0].re * m[0].re + m[0].im * m[0].im + m[1].re * m[1].re + m[1].im * m[1].im
+                           + m[2].re * m[2].re + m[2].im * m[2].im);
+    m[0] = n0 * m[0];
+    m[1] = n0 * m[1];
+    m[2] = n0 * m[2];
+    cd psi = mul_c(m[3], m[0]) + mul_c(m[4], m[1]) + mul_c(m[5], m[2]);
+    m[3] = m[3] - psi * m[0];
+    m[4] = m[4] - psi * m[1];
+    m[5] = m[5] - psi * m[2];
+    double n1 = 1.0 / sqrt(m[3].re * m[3].re + m[3].im * m[3].im + m[4].re * m[4].re + m[4].im * m[4].im
+                           + m[5].re * m[5].re + m[5].im * m[5].im);
+    m[3] = n1 * m[3];
+    m[4] = n1 * m[4];
+    m[5] = n1 * m[5];
+    m[6] = conj(m[1] * m[5] - m[2] * m[4]);
+    m[7] = conj(m[2] * m[3] - m[0] * m[5]);
+    m[8] = conj(m[0] * m[4] - m[1] * m[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Cayley-Hamilton exp(iQ) for traceless hermitian Q (Morningstar-Peardon),
+// with the same branch structure and Taylor switch-over points as the
+// reference's SU3::ExpConstants.
+struct ExpC {
+    double u, w, u2, w2, c0max;
+    int signflip;
+    cd exp_miu, exp_2iu, h0, h1, h2;
+    double denom, cosw, xi0;
+};
+
+__device__ __forceinline__ double xi0_of(double w)
+{
+    if (fabs(w) <= 0.05) {
+        double w2 = w * w;
+        return 1.0 - (1.0 / 6.0) * w2 * (1.0 - (1.0 / 20.0) * w2 * (1.0 - (1.0 / 42.0) * w2));
+    }
+    return sin(w) / w;
+}
+__device__ __forceinline__ double xi1_of(double w)
+{
+    if (fabs(w) <= 0.115) {
+        double w2 = w * w;
+        return -(1.0 / 3.0) + (1.0 / 30.0) * w2 * (1.0 - (1.0 / 28.0) * w2 * (1.0 - (1.0 / 54.0) * w2));
+    }
+    return cos(w) / (w * w) - sin(w) / (w * w * w);
+}
+
+// Q2 = Q*Q must be supplied by the caller (it is needed again for the exponential).
+__device__ __forceinline__ void exp_constants(const M3& Q, const M3& Q2, ExpC& e)
+{
+    // Re tr(Q Q2) and Re tr(Q2)
+    double trq3 = retr_mul(Q, Q2);
+    double c0s = (1.0 / 3.0) * trq3;
+    double c0 = fabs(c0s);
+    double c1 = 0.5 * (Q2.m[0].re + Q2.m[4].re + Q2.m[8].re);
+    e.c0max = 2.0 * pow(c1 / 3.0, 1.5);
+    e.signflip = c0s < 0.0;
+    double theta = acos(fmin(c0 / e.c0max, 1.0));
+    double st, ct;
+    sincos((1.0 / 3.0) * theta, &st, &ct);
+    e.u = sqrt((1.0 / 3.0) * c1) * ct;
+    e.w = sqrt(c1) * st;
+    e.u2 = e.u * e.u;
+    e.w2 = e.w * e.w;
+    double su, cu;
+    sincos(e.u, &su, &cu);
+    e.exp_miu = cd{cu, -su};
+    e.exp_2iu = conj(e.exp_miu * e.exp_miu);
+    e.cosw = cos(e.w);
+    e.xi0 = xi0_of(e.w);
+    if (e.c0max < 2.2250738585072014e-308) {
+        e.denom = 1.0;
+        e.h0 = cd{1.0, 0.0};
+        e.h1 = cd{0.0, 0.0};
+        e.h2 = cd{0.0, 0.0};
+        return;
+    }
+    const double u = e.u, u2 = e.u2, w2 = e.w2;
+    const cd ixi0 = cd{0.0, e.xi0};
+    e.denom = 1.0 / (9.0 * u2 - w2);
+    cd h0 = (u2 - w2) * e.exp_2iu + e.exp_miu * (cd{8.0 * u2 * e.cosw, 0.0} + (2.0 * u) * ixi0 * (3.0 * u2 + w2));
+    cd h1 = (2.0 * u) * e.exp_2iu - e.exp_miu * (cd{2.0 * u * e.cosw, 0.0} - ixi0 * (3.0 * u2 - w2));
+    cd h2 = e.exp_2iu - e.exp_miu * (cd{e.cosw, 0.0} + (3.0 * u) * ixi0);
+    if (e.signflip) {
+        h0 = conj(h0);
+        h1 = -conj(h1);
+        h2 = conj(h2);
+    }
+    e.h0 = h0;
+    e.h1 = h1;
+    e.h2 = h2;
+}
+
+__device__ __forceinline__ M3 exp_from(const ExpC& e, const M3& Q, const M3& Q2)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.m[i] = e.denom * (e.h1 * Q.m[i] + e.h2 * Q2.m[i]);
+    r.m[0] += e.denom * e.h0;
+    r.m[4] += e.denom * e.h0;
+    r.m[8] += e.denom * e.h0;
+    return r;
+}
+
+// exp(iQ)
+__device__ __forceinline__ M3 su3_exp(const M3& Q)
+{
+    M3 Q2 = mul(Q, Q);
+    ExpC e;
+    exp_constants(Q, Q2, e);
+    return exp_from(e, Q, Q2);
+}
+
+// exp(c * A) for traceless anti-hermitian A: exp(i * (-i c A))
+__device__ __forceinline__ M3 exp_ah(const M3& A, double c)
+{
+    M3 Q;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Q.m[i] = cd{c * A.m[i].im, -c * A.m[i].re};   // -i c A
+    return su3_exp(Q);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 counter RNG (Salmon et al., SC'11).  Counter layout:
+//   key = seed, ctr = {site, stream<<24 | k<<4 | sub<<2 | mu, counter_lo, counter_hi}
+enum : uint32_t { RNG_HEATBATH = 0, RNG_MOMENTUM = 1, RNG_HAAR = 2 };
+
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+    }
+}
+
+__device__ __forceinline__ double u01(uint32_t a, uint32_t b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ void draw2(uint64_t seed, uint32_t site, uint32_t stream, uint32_t k, uint32_t sub,
+                                      uint32_t mu, uint64_t counter, double& u0, double& u1)
+{
+    uint32_t c[4] = {site, (stream << 24) | (k << 4) | (sub << 2) | mu, (uint32_t)counter, (uint32_t)(counter >> 32)};
+    philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    u0 = u01(c[0], c[1]);
+    u1 = u01(c[2], c[3]);
+}
+
+// Box-Muller pair from two uniforms
+__device__ __forceinline__ void gauss2(double u0, double u1, double& g0, double& g1)
+{
+    double r = sqrt(-2.0 * log(1.0 - u0));
+    double s, c;
+    sincos(2.0 * 3.14159265358979323846 * u1, &s, &c);
+    g0 = r * c;
+    g1 = r * s;
+}
+
+// ---------------------------------------------------------------------------
+// Lattice geometry.  Sites are lexicographic s = ((t*Nx+x)*Ny+y)*Nz+z; the parity
+// of a site is (t+x+y+z)&1 and, because Nz is even, its checkerboard index is s>>1.
+// Links live in a structure-of-arrays buffer of complex doubles:
+//   U[((mu*2 + parity)*9 + c)*Vh + cb],  c = 3*row + col,  Vh = V/2.
+struct Geom {
+    int L[4];       // Nt, Nx, Ny, Nz
+    int st[4];      // lexicographic strides of t, x, y, z
+    int V, Vh;
+};
+
+__device__ __forceinline__ void coords_of(const Geom& g, int s, int c[4])
+{
+    c[3] = s % g.L[3];
+    int r = s / g.L[3];
+    c[2] = r % g.L[2];
+    r /= g.L[2];
+    c[1] = r % g.L[1];
+    c[0] = r / g.L[1];
+}
+// site index one step along +mu / -mu given the coordinate of that direction
+__device__ __forceinline__ int up(const Geom& g, int s, int cmu, int mu)
+{
+    return (cmu + 1 < g.L[mu]) ? s + g.st[mu] : s - (g.L[mu] - 1) * g.st[mu];
+}
+__device__ __forceinline__ int dn(const Geom& g, int s, int cmu, int mu)
+{
+    return (cmu > 0) ? s - g.st[mu] : s + (g.L[mu] - 1) * g.st[mu];
+}
+
+__device__ __forceinline__ M3 ld(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
+{
+    const double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    M3 r;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        double2 v = p[(size_t)c * g.Vh];
+        r.m[c] = cd{v.x, v.y};
+    }
+    return r;
+}
+__device__ __forceinline__ void st_link(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
+{
+    double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) p[(size_t)c * g.Vh] = make_double2(a.m[c].re, a.m[c].im);
+}
+__device__ __forceinline__ int parity_of(const int c[4]) { return (c[0] + c[1] + c[2] + c[3]) & 1; }
+
+// Site of the cb-th entry of a parity: s = 2*cb or 2*cb+1 with the right parity.
+__device__ __forceinline__ int site_of_cb(const Geom& g, int cb, int par, int c[4])
+{
+    int s = 2 * cb;
+    coords_of(g, s, c);
+    int flip = (c[0] + c[1] + c[2] + par) & 1;   // z of s is even
+    c[3] += flip;
+    return s + flip;
+}
+
+// Staple of link (s, mu): sum over nu != mu (ascending) of
+//   U_nu(x) U_mu(x+nu) U_nu(x+mu)^+ + U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu)
+__device__ __forceinline__ M3 staple(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu)
+{
+    const int par = (c[0] + c[1] + c[2] + c[3]) & 1;
+    const int s_mu = up(g, s, c[mu], mu);
+    M3 st = m3_zero();
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        const int s_nu = up(g, s, c[nu], nu);
+        const int s_nd = dn(g, s, c[nu], nu);
+        const int s_nd_mu = up(g, s_nd, c[mu], mu);
+        {
+            M3 a = ld(U, g, s, nu, par);
+            M3 b = ld(U, g, s_nu, mu, par ^ 1);
+            M3 ab = mul(a, b);
+            M3 cc = ld(U, g, s_mu, nu, par ^ 1);
+            acc(st, mul_nd(ab, cc));
+        }
+        {
+            M3 a = ld(U, g, s_nd, nu, par ^ 1);
+            M3 b = ld(U, g, s_nd, mu, par ^ 1);
+            M3 ab = mul_dn(a, b);
+            M3 cc = ld(U, g, s_nd_mu, nu, par);
+            acc(st, mul(ab, cc));
+        }
+    }
+    return st;
+}
+
+}  // namespace lt

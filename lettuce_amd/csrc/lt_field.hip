@@ -1,0 +1,175 @@
+// lt_field.hip -- layout conversion (reference AoS <-> even/odd SoA), field
+// initialisation (identity, Haar-random), reunitarisation and health checks.
+#include "lt_internal.hpp"
+
+namespace lt {
+
+namespace {
+
+constexpr int BS = 256;
+
+// one thread per (site, mu): AoS link ((s*4+mu)*18 doubles) <-> SoA slots
+__global__ void __launch_bounds__(BS) pack_kernel(double2* __restrict__ U, const double* __restrict__ aos, Geom g)
+{
+    long long id = (long long)blockIdx.x * BS + threadIdx.x;
+    if (id >= 4LL * g.V) return;
+    int s = (int)(id >> 2), mu = (int)(id & 3);
+    int c[4];
+    coords_of(g, s, c);
+    const double* src = aos + id * 18;
+    M3 a;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.m[k] = cd{src[2 * k], src[2 * k + 1]};
+    st_link(U, g, s, mu, parity_of(c), a);
+}
+
+__global__ void __launch_bounds__(BS) unpack_kernel(const double2* __restrict__ U, double* __restrict__ aos, Geom g)
+{
+    long long id = (long long)blockIdx.x * BS + threadIdx.x;
+    if (id >= 4LL * g.V) return;
+    int s = (int)(id >> 2), mu = (int)(id & 3);
+    int c[4];
+    coords_of(g, s, c);
+    M3 a = ld(U, g, s, mu, parity_of(c));
+    double* dst = aos + id * 18;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        dst[2 * k] = a.m[k].re;
+        dst[2 * k + 1] = a.m[k].im;
+    }
+}
+
+// one thread per SoA slot group: (mu, parity, cb)
+__global__ void __launch_bounds__(BS) identity_kernel(double2* __restrict__ U, Geom g)
+{
+    int cb = blockIdx.x * BS + threadIdx.x;
+    if (cb >= g.Vh) return;
+    int mp = blockIdx.y;   // mu*2 + parity
+    double2* p = U + (size_t)mp * 9 * g.Vh + cb;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) p[(size_t)c * g.Vh] = make_double2((c % 4 == 0) ? 1.0 : 0.0, 0.0);
+}
+
+// Haar SU(3): complex Gaussian rows 0 and 1 (Philox stream RNG_HAAR, draw block k
+// gives entry k), Gram-Schmidt, row 2 = conj(row0 x row1)
+__global__ void __launch_bounds__(BS) haar_kernel(double2* __restrict__ U, Geom g, uint64_t seed)
+{
+    int cb = blockIdx.x * BS + threadIdx.x;
+    if (cb >= g.Vh) return;
+    int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
+    int c[4];
+    int s = site_of_cb(g, cb, par, c);
+    M3 a = m3_zero();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double u0, u1, g0, g1;
+        draw2(seed, (uint32_t)s, RNG_HAAR, (uint32_t)k, 0u, (uint32_t)mu, 0ull, u0, u1);
+        gauss2(u0, u1, g0, g1);
+        a.m[k] = cd{g0, g1};
+    }
+    gram_schmidt(a);
+    st_link(U, g, s, mu, par, a);
+}
+
+__global__ void __launch_bounds__(BS) reunit_kernel(double2* __restrict__ U, Geom g)
+{
+    int cb = blockIdx.x * BS + threadIdx.x;
+    if (cb >= g.Vh) return;
+    int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
+    int c[4];
+    int s = site_of_cb(g, cb, par, c);
+    M3 a = ld(U, g, s, mu, par);
+    gram_schmidt(a);
+    st_link(U, g, s, mu, par, a);
+}
+
+// per-block max |U^+U - 1| (Frobenius, per link) and count of non-finite entries
+__global__ void __launch_bounds__(BS) check_kernel(const double2* __restrict__ U, Geom g, double* part)
+{
+    __shared__ double smax[BS];
+    __shared__ double scnt[BS];
+    int cb = blockIdx.x * BS + threadIdx.x;
+    double dev = 0.0, bad = 0.0;
+    if (cb < g.Vh) {
+        int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
+        int c[4];
+        int s = site_of_cb(g, cb, par, c);
+        M3 a = ld(U, g, s, mu, par);
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            if (!isfinite(a.m[k].re) || !isfinite(a.m[k].im)) bad += 1.0;
+        M3 p = mul_dn(a, a);
+        p.m[0].re -= 1.0;
+        p.m[4].re -= 1.0;
+        p.m[8].re -= 1.0;
+        double f = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f += p.m[k].re * p.m[k].re + p.m[k].im * p.m[k].im;
+        dev = sqrt(f);
+        if (!(dev == dev)) dev = 1.0e300;
+    }
+    smax[threadIdx.x] = dev;
+    scnt[threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+            scnt[threadIdx.x] += scnt[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        part[2 * b] = smax[0];
+        part[2 * b + 1] = scnt[0];
+    }
+}
+
+}  // namespace
+
+void k_pack(lt_field f, const double* d_aos)
+{
+    launch(f->ctx, "pack", pack_kernel, dim3(nblocks(4LL * f->g.V, BS)), dim3(BS), 0, f->d, d_aos, f->g);
+}
+
+void k_unpack(lt_field f, double* d_aos)
+{
+    launch(f->ctx, "unpack", unpack_kernel, dim3(nblocks(4LL * f->g.V, BS)), dim3(BS), 0, (const double2*)f->d, d_aos,
+           f->g);
+}
+
+void k_identity(lt_field f)
+{
+    launch(f->ctx, "identity", identity_kernel, dim3(nblocks(f->g.Vh, BS), 8), dim3(BS), 0, f->d, f->g);
+}
+
+void k_haar(lt_field f, uint64_t seed)
+{
+    launch(f->ctx, "haar", haar_kernel, dim3(nblocks(f->g.Vh, BS), 8), dim3(BS), 0, f->d, f->g, seed);
+}
+
+void k_reunitarize(lt_field f)
+{
+    launch(f->ctx, "reunitarize", reunit_kernel, dim3(nblocks(f->g.Vh, BS), 8), dim3(BS), 0, f->d, f->g);
+}
+
+void k_check(lt_field f, double* max_dev, long long* nonfinite)
+{
+    lt_ctx ctx = f->ctx;
+    unsigned nb = nblocks(f->g.Vh, BS);
+    size_t np = (size_t)nb * 8;
+    double* part = ctx->scratch_for(2 * np);
+    launch(ctx, "check", check_kernel, dim3(nb, 8), dim3(BS), 0, (const double2*)f->d, f->g, part);
+    double* h = ctx->host_for(2 * np);
+    LT_HIP_CHECK(hipMemcpyAsync(h, part, 2 * np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double m = 0.0, n = 0.0;
+    for (size_t i = 0; i < np; ++i) {
+        m = h[2 * i] > m ? h[2 * i] : m;
+        n += h[2 * i + 1];
+    }
+    *max_dev = m;
+    *nonfinite = (long long)n;
+}
+
+}  // namespace lt

