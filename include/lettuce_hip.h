@@ -87,6 +87,11 @@ int lt_field_check(lt_field f, double *max_unitarity_dev, long long *n_nonfinite
 /* counter-based RNG state of the field (replaces PRNG4D, random/prng_wrapper.hpp:43-379) */
 int lt_field_set_rng(lt_field f, uint64_t seed, uint64_t counter);
 int lt_field_get_rng(lt_field f, uint64_t *seed, uint64_t *counter);
+/* launch shape of the link kernels: thread blocks cover 4D tiles of extent
+ * (tt, tx, ty, tz) in (t, x, y, z/2), which must divide the lattice; results do
+ * not depend on it (no reference counterpart: the reference's OpenMP schedule) */
+int lt_field_set_tile(lt_field f, int tt, int tx, int ty, int tz);
+int lt_field_get_tile(lt_field f, int tile[4]);
 
 /* ---- actions and local updates ------------------------------------------- */
 /* Rectangular<1>::Staple / WilsonAction::Staple for every link into out

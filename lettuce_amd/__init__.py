@@ -135,6 +135,15 @@ class GaugeField:
     def set_rng(self, seed, counter=0):
         call("lt_field_set_rng", self._h, seed, counter)
 
+    def set_tile(self, tt, tx, ty, tz):
+        """Thread-block tile (t, x, y, z/2) of the link kernels; does not change results."""
+        call("lt_field_set_tile", self._h, tt, tx, ty, tz)
+
+    def get_tile(self):
+        t = (ctypes.c_int * 4)()
+        call("lt_field_get_tile", self._h, t)
+        return tuple(t)
+
     def get_rng(self):
         s = ctypes.c_uint64()
         c = ctypes.c_uint64()

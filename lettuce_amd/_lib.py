@@ -37,6 +37,8 @@ SIGNATURES = {
     "lt_field_check": [c_field, D, ctypes.POINTER(ctypes.c_longlong)],
     "lt_field_set_rng": [c_field, ctypes.c_uint64, ctypes.c_uint64],
     "lt_field_get_rng": [c_field, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
+    "lt_field_set_tile": [c_field, I, I, I, I],
+    "lt_field_get_tile": [c_field, ctypes.POINTER(I)],
     "lt_staple_field": [c_field, c_field],
     "lt_or_sweep": [c_field, I, I],
     "lt_heatbath_sweep": [c_field, ctypes.c_double, I, I],

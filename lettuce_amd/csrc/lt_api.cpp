@@ -31,6 +31,76 @@ Geom make_geom(int nt, int nx, int ny, int nz)
     return g;
 }
 
+bool make_linkmap(const Geom& g, int tt, int tx, int ty, int tz, LinkMap& m)
+{
+    const int ext[4] = {g.L[0], g.L[1], g.L[2], g.L[3] / 2};
+    const int T[4] = {tt, tx, ty, tz};
+    int n = 1;
+    for (int i = 0; i < 4; ++i) {
+        if (T[i] < 1 || ext[i] % T[i]) return false;
+        n *= T[i];
+    }
+    if (n < 1 || n > LT_LINK_BS) return false;
+    for (int i = 0; i < 4; ++i) {
+        m.T[i] = T[i];
+        m.N[i] = ext[i] / T[i];
+    }
+    m.nthreads = n;
+    return true;
+}
+
+// Default tile: 128 threads; z (checkerboard) fastest, filled up to 16 entries,
+// then y, x, t (measured best at 32^4: whole z rows, 8 y rows).
+LinkMap default_linkmap(const Geom& g)
+{
+    LinkMap m{};
+    const int ext[4] = {g.L[0], g.L[1], g.L[2], g.L[3] / 2};
+    int T[4] = {1, 1, 1, 1};
+    int want = 128;
+    const int order[4] = {3, 2, 1, 0};
+    for (int k = 0; k < 4 && want > 1; ++k) {
+        int i = order[k];
+        int lim = (i == 3) ? 16 : ext[i];
+        while (want > 1 && T[i] * 2 <= lim && ext[i] % (T[i] * 2) == 0) {
+            T[i] *= 2;
+            want /= 2;
+        }
+    }
+    for (int k = 0; k < 4 && want > 1; ++k) {   // second pass without the z/y caps
+        int i = order[k];
+        while (want > 1 && ext[i] % (T[i] * 2) == 0) {
+            T[i] *= 2;
+            want /= 2;
+        }
+    }
+    if (want == 1 && make_linkmap(g, T[0], T[1], T[2], T[3], m)) return m;
+    // extents without enough factors of two: the largest dividing tile of at most
+    // 256 threads, z-major
+    int best = 0, bt[4] = {1, 1, 1, 1};
+    for (int tz = 1; tz <= ext[3]; ++tz) {
+        if (ext[3] % tz) continue;
+        for (int ty = 1; ty <= ext[2]; ++ty) {
+            if (ext[2] % ty) continue;
+            for (int tx = 1; tx <= ext[1]; ++tx) {
+                if (ext[1] % tx) continue;
+                for (int tt = 1; tt <= ext[0]; ++tt) {
+                    if (ext[0] % tt) continue;
+                    int n = tt * tx * ty * tz;
+                    if (n <= LT_LINK_BS && n > best) {
+                        best = n;
+                        bt[0] = tt;
+                        bt[1] = tx;
+                        bt[2] = ty;
+                        bt[3] = tz;
+                    }
+                }
+            }
+        }
+    }
+    make_linkmap(g, bt[0], bt[1], bt[2], bt[3], m);
+    return m;
+}
+
 }  // namespace lt
 
 double* lt_ctx_s::scratch_for(size_t n)
@@ -215,6 +285,7 @@ int lt_field_create(lt_ctx ctx, int nt, int nx, int ny, int nz, lt_field* out)
         auto* f = new lt_field_s();
         f->ctx = ctx;
         f->g = lt::make_geom(nt, nx, ny, nz);
+        f->map = lt::default_linkmap(f->g);
         try {
             f->d = alloc_links(f->g);
         } catch (...) {
@@ -257,6 +328,7 @@ int lt_field_upload(lt_field f, const double* aos)
         if (e == hipSuccess) {
             try {
                 lt::k_pack(f, stage);
+                f->su3 = true;
             } catch (...) {
                 (void)hipStreamSynchronize(f->ctx->stream);
                 (void)hipFree(stage);
@@ -301,7 +373,8 @@ int lt_field_copy(lt_field dst, lt_field src)
     if (dst == src) return LT_OK;
     return guarded([&] {
         use_device(dst->ctx);
-        LT_HIP_CHECK(hipMemcpyAsync(dst->d, src->d, dst->bytes(), hipMemcpyDeviceToDevice, dst->ctx->stream));
+        lt::k_copy(dst->ctx, src->d, dst->d, (size_t)72 * dst->g.Vh);
+        dst->su3 = src->su3;
         return LT_OK;
     });
 }
@@ -311,6 +384,7 @@ int lt_field_swap(lt_field a, lt_field b)
     if (!a || !b) return fail(LT_EINVAL, "lt_field_swap: null argument");
     if (!same_shape(a, b) || a->ctx != b->ctx) return fail(LT_ESTATE, "lt_field_swap: shape/context mismatch");
     std::swap(a->d, b->d);
+    std::swap(a->su3, b->su3);
     return LT_OK;
 }
 
@@ -320,6 +394,7 @@ int lt_field_set_identity(lt_field f)
     return guarded([&] {
         use_device(f->ctx);
         lt::k_identity(f);
+        f->su3 = true;
         return LT_OK;
     });
 }
@@ -330,6 +405,7 @@ int lt_field_set_haar(lt_field f, uint64_t seed)
     return guarded([&] {
         use_device(f->ctx);
         lt::k_haar(f, seed);
+        f->su3 = true;
         return LT_OK;
     });
 }
@@ -362,6 +438,23 @@ int lt_field_set_rng(lt_field f, uint64_t seed, uint64_t counter)
     return LT_OK;
 }
 
+int lt_field_set_tile(lt_field f, int tt, int tx, int ty, int tz)
+{
+    if (!f) return fail(LT_EINVAL, "null field");
+    lt::LinkMap m{};
+    if (!lt::make_linkmap(f->g, tt, tx, ty, tz, m))
+        return fail(LT_EINVAL, "lt_field_set_tile: tile must divide (Nt, Nx, Ny, Nz/2) with 1..1024 threads");
+    f->map = m;
+    return LT_OK;
+}
+
+int lt_field_get_tile(lt_field f, int tile[4])
+{
+    if (!f || !tile) return fail(LT_EINVAL, "null argument");
+    for (int i = 0; i < 4; ++i) tile[i] = f->map.T[i];
+    return LT_OK;
+}
+
 int lt_field_get_rng(lt_field f, uint64_t* seed, uint64_t* counter)
 {
     if (!f) return fail(LT_EINVAL, "null field");
@@ -377,6 +470,7 @@ int lt_staple_field(lt_field u, lt_field out)
     return guarded([&] {
         use_device(u->ctx);
         lt::k_staple_field(u, out);
+        out->su3 = false;
         return LT_OK;
     });
 }
@@ -494,8 +588,7 @@ int lt_stout(lt_field in, lt_field out, double rho, int n_levels)
         use_device(in->ctx);
         lt_ctx ctx = in->ctx;
         if (n_levels == 0) {
-            if (in != out)
-                LT_HIP_CHECK(hipMemcpyAsync(out->d, in->d, in->bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+            if (in != out) lt::k_copy(ctx, in->d, out->d, (size_t)72 * in->g.Vh);
             return LT_OK;
         }
         // ping-pong so that the last level lands in out->d
@@ -507,11 +600,10 @@ int lt_stout(lt_field in, lt_field out, double rho, int n_levels)
             if (last) dst = (src == out->d) ? tmp : out->d;
             else dst = (src == tmp) ? out->d : tmp;
             if (dst == src) dst = (dst == tmp) ? out->d : tmp;
-            lt::k_stout(ctx, in->g, src, dst, rho);
+            lt::k_stout(ctx, in->g, in->map, src, dst, rho);
             src = dst;
         }
-        if (src != out->d)
-            LT_HIP_CHECK(hipMemcpyAsync(out->d, src, in->bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+        if (src != out->d) lt::k_copy(ctx, src, out->d, (size_t)72 * in->g.Vh);
         LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         LT_HIP_CHECK(hipFree(tmp));
         return LT_OK;
@@ -565,7 +657,7 @@ static void flow_steps(lt_flow fl, int n)
     lt_ctx ctx = u->ctx;
     const double e = fl->eps;
     auto stage = [&](double c_old, double c_new, double c_exp, int read_z, int write_z) {
-        lt::k_flow_stage(ctx, u->g, u->d, fl->tmp, fl->z, c_old, c_new, c_exp, read_z, write_z);
+        lt::k_flow_stage(ctx, u->g, u->map, u->d, fl->tmp, fl->z, c_old, c_new, c_exp, read_z, write_z);
         std::swap(u->d, fl->tmp);
     };
     for (int k = 0; k < n; ++k) {
@@ -584,8 +676,7 @@ int lt_flow_run(lt_flow fl, int n_steps)
     if (!fl || n_steps < 0) return fail(LT_EINVAL, "lt_flow_run: bad argument");
     return guarded([&] {
         use_device(fl->flowed->ctx);
-        LT_HIP_CHECK(hipMemcpyAsync(fl->flowed->d, fl->unflowed->d, fl->flowed->bytes(), hipMemcpyDeviceToDevice,
-                                    fl->flowed->ctx->stream));
+        lt::k_copy(fl->flowed->ctx, fl->unflowed->d, fl->flowed->d, (size_t)72 * fl->flowed->g.Vh);
         flow_steps(fl, n_steps);
         return LT_OK;
     });

@@ -9,7 +9,32 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Launch-shape knobs of the link-update kernels (sweeps, stout, flow stages).
+// LT_LINK_BS is the largest thread block they are compiled for; the host never
+// launches them with more threads (tiles are validated against it).
+#ifndef LT_LINK_BS
+#define LT_LINK_BS 256
+#endif
+#ifndef LT_LINK_WAVES
+#define LT_LINK_WAVES 2
+#endif
+#ifndef LT_XCD_REMAP
+#define LT_XCD_REMAP 1
+#endif
+#define LT_LINK_KERNEL __launch_bounds__(LT_LINK_BS) __attribute__((amdgpu_waves_per_eu(LT_LINK_WAVES)))
+
 namespace lt {
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X, each with
+// its own L2.  Remapping the block index so that every XCD works on one
+// contiguous range of sites keeps the z/y neighbours of a block in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+#if LT_XCD_REMAP
+    if ((nb & 7) == 0) return (b & 7) * (nb >> 3) + (b >> 3);
+#endif
+    return b;
+}
 
 // ---------------------------------------------------------------------------
 // complex double
@@ -401,7 +426,72 @@ __device__ __forceinline__ void st_link(double2* __restrict__ U, const Geom& g, 
 #pragma unroll
     for (int c = 0; c < 9; ++c) p[(size_t)c * g.Vh] = make_double2(a.m[c].re, a.m[c].im);
 }
+// SU(3) links are stored as rows 0 and 1 only: every update ends with the
+// Gram-Schmidt step whose last row is conj(row0 x row1), so row 2 is rebuilt on
+// load with exactly that formula and never travels through memory.
+__device__ __forceinline__ void third_row(M3& a)
+{
+    cd* m = a.m;
+    m[6] = conj(m[1] * m[5] - m[2] * m[4]);
+    m[7] = conj(m[2] * m[3] - m[0] * m[5]);
+    m[8] = conj(m[0] * m[4] - m[1] * m[3]);
+}
+__device__ __forceinline__ M3 ldu(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
+{
+    const double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    M3 r;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double2 v = p[(size_t)c * g.Vh];
+        r.m[c] = cd{v.x, v.y};
+    }
+    third_row(r);
+    return r;
+}
+__device__ __forceinline__ void stu(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
+{
+    double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) p[(size_t)c * g.Vh] = make_double2(a.m[c].re, a.m[c].im);
+}
+
 __device__ __forceinline__ int parity_of(const int c[4]) { return (c[0] + c[1] + c[2] + c[3]) & 1; }
+
+// Thread -> site map of the per-parity link kernels.  Blocks cover 4D tiles of
+// (t, x, y, zc) with zc = z/2 the checkerboard coordinate, z fastest inside a tile
+// and tiles ordered z, y, x, t; the block index is first spread over XCDs.
+struct LinkMap {
+    int T[4];     // tile extent in t, x, y, zc
+    int N[4];     // number of tiles in t, x, y, zc
+    int nthreads; // T[0]*T[1]*T[2]*T[3] == blockDim.x
+};
+
+__device__ __forceinline__ bool map_site(const Geom& g, const LinkMap& m, int par, int c[4], int& s)
+{
+    int b = xcd_block(blockIdx.x, gridDim.x);
+    int tid = threadIdx.x;
+    int bz = b % m.N[3];
+    b /= m.N[3];
+    int by = b % m.N[2];
+    b /= m.N[2];
+    int bx = b % m.N[1];
+    int bt = b / m.N[1];
+    if (bt >= m.N[0] || tid >= m.nthreads) return false;
+    int dz = tid % m.T[3];
+    tid /= m.T[3];
+    int dy = tid % m.T[2];
+    tid /= m.T[2];
+    int dx = tid % m.T[1];
+    int dt = tid / m.T[1];
+    c[0] = bt * m.T[0] + dt;
+    c[1] = bx * m.T[1] + dx;
+    c[2] = by * m.T[2] + dy;
+    int zc = bz * m.T[3] + dz;
+    int flip = (c[0] + c[1] + c[2] + par) & 1;
+    c[3] = 2 * zc + flip;
+    s = ((c[0] * g.L[1] + c[1]) * g.L[2] + c[2]) * g.L[3] + c[3];
+    return true;
+}
 
 // Site of the cb-th entry of a parity: s = 2*cb or 2*cb+1 with the right parity.
 __device__ __forceinline__ int site_of_cb(const Geom& g, int cb, int par, int c[4])
@@ -427,17 +517,17 @@ __device__ __forceinline__ M3 staple(const double2* __restrict__ U, const Geom& 
         const int s_nd = dn(g, s, c[nu], nu);
         const int s_nd_mu = up(g, s_nd, c[mu], mu);
         {
-            M3 a = ld(U, g, s, nu, par);
-            M3 b = ld(U, g, s_nu, mu, par ^ 1);
+            M3 a = ldu(U, g, s, nu, par);
+            M3 b = ldu(U, g, s_nu, mu, par ^ 1);
             M3 ab = mul(a, b);
-            M3 cc = ld(U, g, s_mu, nu, par ^ 1);
+            M3 cc = ldu(U, g, s_mu, nu, par ^ 1);
             acc(st, mul_nd(ab, cc));
         }
         {
-            M3 a = ld(U, g, s_nd, nu, par ^ 1);
-            M3 b = ld(U, g, s_nd, mu, par ^ 1);
+            M3 a = ldu(U, g, s_nd, nu, par ^ 1);
+            M3 b = ldu(U, g, s_nd, mu, par ^ 1);
             M3 ab = mul_dn(a, b);
-            M3 cc = ld(U, g, s_nd_mu, nu, par);
+            M3 cc = ldu(U, g, s_nd_mu, nu, par);
             acc(st, mul(ab, cc));
         }
     }

@@ -23,14 +23,15 @@ __global__ void __launch_bounds__(BS) pack_kernel(double2* __restrict__ U, const
     st_link(U, g, s, mu, parity_of(c), a);
 }
 
-__global__ void __launch_bounds__(BS) unpack_kernel(const double2* __restrict__ U, double* __restrict__ aos, Geom g)
+__global__ void __launch_bounds__(BS) unpack_kernel(const double2* __restrict__ U, double* __restrict__ aos, Geom g,
+                                                    int su3)
 {
     long long id = (long long)blockIdx.x * BS + threadIdx.x;
     if (id >= 4LL * g.V) return;
     int s = (int)(id >> 2), mu = (int)(id & 3);
     int c[4];
     coords_of(g, s, c);
-    M3 a = ld(U, g, s, mu, parity_of(c));
+    M3 a = su3 ? ldu(U, g, s, mu, parity_of(c)) : ld(U, g, s, mu, parity_of(c));
     double* dst = aos + id * 18;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(BS) haar_kernel(double2* __restrict__ U, Geom 
         a.m[k] = cd{g0, g1};
     }
     gram_schmidt(a);
-    st_link(U, g, s, mu, par, a);
+    stu(U, g, s, mu, par, a);
 }
 
 __global__ void __launch_bounds__(BS) reunit_kernel(double2* __restrict__ U, Geom g)
@@ -80,11 +81,11 @@ __global__ void __launch_bounds__(BS) reunit_kernel(double2* __restrict__ U, Geo
     int s = site_of_cb(g, cb, par, c);
     M3 a = ld(U, g, s, mu, par);
     gram_schmidt(a);
-    st_link(U, g, s, mu, par, a);
+    stu(U, g, s, mu, par, a);
 }
 
 // per-block max |U^+U - 1| (Frobenius, per link) and count of non-finite entries
-__global__ void __launch_bounds__(BS) check_kernel(const double2* __restrict__ U, Geom g, double* part)
+__global__ void __launch_bounds__(BS) check_kernel(const double2* __restrict__ U, Geom g, int su3, double* part)
 {
     __shared__ double smax[BS];
     __shared__ double scnt[BS];
@@ -94,7 +95,7 @@ __global__ void __launch_bounds__(BS) check_kernel(const double2* __restrict__ U
         int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
         int c[4];
         int s = site_of_cb(g, cb, par, c);
-        M3 a = ld(U, g, s, mu, par);
+        M3 a = su3 ? ldu(U, g, s, mu, par) : ld(U, g, s, mu, par);
 #pragma unroll
         for (int k = 0; k < 9; ++k)
             if (!isfinite(a.m[k].re) || !isfinite(a.m[k].im)) bad += 1.0;
@@ -125,7 +126,22 @@ __global__ void __launch_bounds__(BS) check_kernel(const double2* __restrict__ U
     }
 }
 
+// plain streaming copy of a link buffer (16-byte elements)
+__global__ void __launch_bounds__(BS) copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * BS;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n)
+{
+    unsigned nb = nblocks((long long)n, BS);
+    if (nb > 65536u) nb = 65536u;
+    launch(ctx, "copy_links", copy_kernel, dim3(nb), dim3(BS), 0, src, dst, n);
+}
 
 void k_pack(lt_field f, const double* d_aos)
 {
@@ -135,7 +151,7 @@ void k_pack(lt_field f, const double* d_aos)
 void k_unpack(lt_field f, double* d_aos)
 {
     launch(f->ctx, "unpack", unpack_kernel, dim3(nblocks(4LL * f->g.V, BS)), dim3(BS), 0, (const double2*)f->d, d_aos,
-           f->g);
+           f->g, (int)f->su3);
 }
 
 void k_identity(lt_field f)
@@ -159,7 +175,7 @@ void k_check(lt_field f, double* max_dev, long long* nonfinite)
     unsigned nb = nblocks(f->g.Vh, BS);
     size_t np = (size_t)nb * 8;
     double* part = ctx->scratch_for(2 * np);
-    launch(ctx, "check", check_kernel, dim3(nb, 8), dim3(BS), 0, (const double2*)f->d, f->g, part);
+    launch(ctx, "check", check_kernel, dim3(nb, 8), dim3(BS), 0, (const double2*)f->d, f->g, (int)f->su3, part);
     double* h = ctx->host_for(2 * np);
     LT_HIP_CHECK(hipMemcpyAsync(h, part, 2 * np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));

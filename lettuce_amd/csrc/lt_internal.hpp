@@ -58,6 +58,8 @@ struct lt_ctx_s {
 struct lt_field_s {
     lt_ctx ctx = nullptr;
     lt::Geom g{};
+    lt::LinkMap map{};         // thread -> site tiling of the per-parity link kernels
+    bool su3 = true;           // rows 0,1 stored, row 2 implied (false: general 3x3 matrices)
     double2* d = nullptr;      // 4 * 2 * 9 * Vh complex doubles
     uint64_t seed = 0;
     uint64_t counter = 0;
@@ -102,10 +104,21 @@ void launch(lt_ctx ctx, const char* name, Kernel kernel, dim3 grid, dim3 block, 
 }
 
 inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// grid size of a tiled link launch; refuses maps the kernels were not compiled for
+inline unsigned map_blocks(const LinkMap& m)
+{
+    if (m.nthreads < 1 || m.nthreads > LT_LINK_BS || m.T[0] * m.T[1] * m.T[2] * m.T[3] != m.nthreads)
+        throw LtError{LT_ESTATE, "invalid thread-block tile for the link kernels"};
+    return (unsigned)(m.N[0] * m.N[1] * m.N[2] * m.N[3]);
+}
 
 Geom make_geom(int nt, int nx, int ny, int nz);
+// tile map with extents (tt, tx, ty, tz) over (t, x, y, z/2); returns false if they do not divide
+bool make_linkmap(const Geom& g, int tt, int tx, int ty, int tz, LinkMap& m);
+LinkMap default_linkmap(const Geom& g);
 
 // kernel launchers (defined in the .hip translation units)
+void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n);
 void k_pack(lt_field f, const double* d_aos);
 void k_unpack(lt_field f, double* d_aos);
 void k_identity(lt_field f);
@@ -117,8 +130,8 @@ void k_or_sweep(lt_field u, int n, int schedule);
 void k_hb_sweep(lt_field u, double beta, int n, int schedule);
 // fused observables: per-timeslice plaquette sum, raw q, E_clov numerator -> host[3*Nt]
 void k_observables(lt_field u, double* host_out);
-void k_stout(lt_ctx ctx, const Geom& g, const double2* in, double2* out, double rho);
-void k_flow_stage(lt_ctx ctx, const Geom& g, const double2* in, double2* out, double2* z, double c_old,
+void k_stout(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double rho);
+void k_flow_stage(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double2* z, double c_old,
                   double c_new, double c_exp, int read_z, int write_z);
 
 }  // namespace lt

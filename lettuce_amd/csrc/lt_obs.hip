@@ -20,20 +20,20 @@ __device__ __forceinline__ M3 clover_plane(const double2* __restrict__ U, const 
     const int smu = up(g, s, c[mu], mu), snu = up(g, s, c[nu], nu);
     const int smd = dn(g, s, c[mu], mu), snd = dn(g, s, c[nu], nu);
     const int smd_nu = up(g, smd, c[nu], nu), smd_nd = dn(g, smd, c[nu], nu), smu_nd = up(g, snd, c[mu], mu);
-    const M3 Umu = ld(U, g, s, mu, par), Unu = ld(U, g, s, nu, par);
+    const M3 Umu = ldu(U, g, s, mu, par), Unu = ldu(U, g, s, nu, par);
     // leaf I: U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-    M3 l = mul_nd(mul_nd(mul(Umu, ld(U, g, smu, nu, q)), ld(U, g, snu, mu, q)), Unu);
+    M3 l = mul_nd(mul_nd(mul(Umu, ldu(U, g, smu, nu, q)), ldu(U, g, snu, mu, q)), Unu);
     retr_plaq = l.m[0].re + l.m[4].re + l.m[8].re;
     M3 C = l;
     // leaf II: U_nu(x) U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
-    const M3 Umu_md = ld(U, g, smd, mu, q);
-    l = mul(mul_nd(mul_nd(Unu, ld(U, g, smd_nu, mu, par)), ld(U, g, smd, nu, q)), Umu_md);
+    const M3 Umu_md = ldu(U, g, smd, mu, q);
+    l = mul(mul_nd(mul_nd(Unu, ldu(U, g, smd_nu, mu, par)), ldu(U, g, smd, nu, q)), Umu_md);
     acc(C, l);
     // leaf III: U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu)
-    l = mul(mul(mul_nd(dag(Umu_md), ld(U, g, smd_nd, nu, par)), ld(U, g, smd_nd, mu, par)), ld(U, g, snd, nu, q));
+    l = mul(mul(mul_nd(dag(Umu_md), ldu(U, g, smd_nd, nu, par)), ldu(U, g, smd_nd, mu, par)), ldu(U, g, snd, nu, q));
     acc(C, l);
     // leaf IV: U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^+
-    l = mul_nd(mul(mul_dn(ld(U, g, snd, nu, q), ld(U, g, snd, mu, q)), ld(U, g, smu_nd, nu, par)), Umu);
+    l = mul_nd(mul(mul_dn(ldu(U, g, snd, nu, q), ldu(U, g, snd, mu, q)), ldu(U, g, smu_nd, nu, par)), Umu);
     acc(C, l);
     return C;
 }

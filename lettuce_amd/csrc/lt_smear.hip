@@ -12,35 +12,29 @@ namespace lt {
 
 namespace {
 
-constexpr int BS = 128;
-
-__global__ void __launch_bounds__(BS) stout_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout, Geom g,
-                                                   double rho)
+__global__ void LT_LINK_KERNEL stout_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout, Geom g,
+                                            LinkMap m, double rho)
 {
-    int cb = blockIdx.x * BS + threadIdx.x;
-    if (cb >= g.Vh) return;
     int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
-    int c[4];
-    int s = site_of_cb(g, cb, par, c);
+    int c[4], s;
+    if (!map_site(g, m, par, c, s)) return;
     M3 st = staple(Uin, g, s, c, mu);
-    M3 L = ld(Uin, g, s, mu, par);
+    M3 L = ldu(Uin, g, s, mu, par);
     M3 A = algebra(mul_nd(st, L));   // C = 1/2 (A - A^+) - 1/6 tr(A - A^+)
     M3 W = mul(exp_ah(A, rho), L);   // exp(-i rho C) applied as exp(i * (-i rho C))
     gram_schmidt(W);
-    st_link(Uout, g, s, mu, par, W);
+    stu(Uout, g, s, mu, par, W);
 }
 
-__global__ void __launch_bounds__(BS) flow_stage_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout,
-                                                        double2* __restrict__ Z, Geom g, double c_old, double c_new,
-                                                        double c_exp, int read_z, int write_z)
+__global__ void LT_LINK_KERNEL flow_stage_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout,
+                                                 double2* __restrict__ Z, Geom g, LinkMap m, double c_old, double c_new,
+                                                 double c_exp, int read_z, int write_z)
 {
-    int cb = blockIdx.x * BS + threadIdx.x;
-    if (cb >= g.Vh) return;
     int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
-    int c[4];
-    int s = site_of_cb(g, cb, par, c);
+    int c[4], s;
+    if (!map_site(g, m, par, c, s)) return;
     M3 st = staple(Uin, g, s, c, mu);
-    M3 L = ld(Uin, g, s, mu, par);
+    M3 L = ldu(Uin, g, s, mu, par);
     M3 F = algebra(mul_nd(st, L));
     M3 z;
     if (read_z) {
@@ -53,21 +47,21 @@ __global__ void __launch_bounds__(BS) flow_stage_kernel(const double2* __restric
     if (write_z) st_link(Z, g, s, mu, par, z);
     M3 W = mul(exp_ah(z, c_exp), L);
     gram_schmidt(W);
-    st_link(Uout, g, s, mu, par, W);
+    stu(Uout, g, s, mu, par, W);
 }
 
 }  // namespace
 
-void k_stout(lt_ctx ctx, const Geom& g, const double2* in, double2* out, double rho)
+void k_stout(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double rho)
 {
-    launch(ctx, "stout", stout_kernel, dim3(nblocks(g.Vh, BS), 8), dim3(BS), 0, in, out, g, rho);
+    launch(ctx, "stout", stout_kernel, dim3(map_blocks(m), 8), dim3(m.nthreads), 0, in, out, g, m, rho);
 }
 
-void k_flow_stage(lt_ctx ctx, const Geom& g, const double2* in, double2* out, double2* z, double c_old, double c_new,
-                  double c_exp, int read_z, int write_z)
+void k_flow_stage(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double2* z, double c_old,
+                  double c_new, double c_exp, int read_z, int write_z)
 {
-    launch(ctx, "flow_stage", flow_stage_kernel, dim3(nblocks(g.Vh, BS), 8), dim3(BS), 0, in, out, z, g, c_old, c_new,
-           c_exp, read_z, write_z);
+    launch(ctx, "flow_stage", flow_stage_kernel, dim3(map_blocks(m), 8), dim3(m.nthreads), 0, in, out, z, g, m, c_old,
+           c_new, c_exp, read_z, write_z);
 }
 
 }  // namespace lt

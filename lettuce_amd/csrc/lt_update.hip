@@ -12,7 +12,7 @@ namespace lt {
 
 namespace {
 
-constexpr int BS = 128;
+constexpr int BS = LT_LINK_BS;
 
 struct SU2 {
     cd e11, e12;
@@ -103,7 +103,7 @@ __device__ __forceinline__ void update_link(double2* __restrict__ U, const Geom&
                                             int par, double prefactor, uint64_t seed, uint64_t counter)
 {
     M3 st = staple(U, g, s, c, mu);
-    M3 L = ld(U, g, s, mu, par);
+    M3 L = ldu(U, g, s, mu, par);
 #pragma unroll
     for (int sub = 0; sub < 3; ++sub) {
         const int i = sub == 2 ? 1 : 0;
@@ -117,25 +117,23 @@ __device__ __forceinline__ void update_link(double2* __restrict__ U, const Geom&
         embed_mul(L, X, i, j);
     }
     gram_schmidt(L);
-    st_link(U, g, s, mu, par, L);
+    stu(U, g, s, mu, par, L);
 }
 
 template <bool HEATBATH>
-__global__ void __launch_bounds__(BS) sweep_cb2_kernel(double2* __restrict__ U, Geom g, int mu, int par,
-                                                        double prefactor, uint64_t seed, uint64_t counter)
+__global__ void LT_LINK_KERNEL sweep_cb2_kernel(double2* __restrict__ U, Geom g, LinkMap m, int mu, int par,
+                                                 double prefactor, uint64_t seed, uint64_t counter)
 {
-    int cb = blockIdx.x * BS + threadIdx.x;
-    if (cb >= g.Vh) return;
-    int c[4];
-    int s = site_of_cb(g, cb, par, c);
+    int c[4], s;
+    if (!map_site(g, m, par, c, s)) return;
     update_link<HEATBATH>(U, g, s, c, mu, par, prefactor, seed, counter);
 }
 
 template <bool HEATBATH>
-__global__ void __launch_bounds__(BS) sweep_cb4_kernel(double2* __restrict__ U, Geom g, int mu, int colour,
-                                                        double prefactor, uint64_t seed, uint64_t counter)
+__global__ void LT_LINK_KERNEL sweep_cb4_kernel(double2* __restrict__ U, Geom g, int mu, int colour,
+                                                 double prefactor, uint64_t seed, uint64_t counter)
 {
-    int s = blockIdx.x * BS + threadIdx.x;
+    int s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (s >= g.V) return;
     int c[4];
     coords_of(g, s, c);
@@ -143,13 +141,12 @@ __global__ void __launch_bounds__(BS) sweep_cb4_kernel(double2* __restrict__ U, 
     update_link<HEATBATH>(U, g, s, c, mu, parity_of(c), prefactor, seed, counter);
 }
 
-__global__ void __launch_bounds__(BS) staple_field_kernel(const double2* __restrict__ U, double2* __restrict__ out, Geom g)
+__global__ void LT_LINK_KERNEL staple_field_kernel(const double2* __restrict__ U, double2* __restrict__ out, Geom g,
+                                                   LinkMap m)
 {
-    int cb = blockIdx.x * BS + threadIdx.x;
-    if (cb >= g.Vh) return;
     int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
-    int c[4];
-    int s = site_of_cb(g, cb, par, c);
+    int c[4], s;
+    if (!map_site(g, m, par, c, s)) return;
     st_link(out, g, s, mu, par, staple(U, g, s, c, mu));
 }
 
@@ -166,9 +163,10 @@ void sweeps(lt_field u, int n, int schedule, double prefactor, const char* name)
                     launch(ctx, name, sweep_cb4_kernel<HEATBATH>, dim3(nblocks(g.V, BS)), dim3(BS), 0, u->d, g, mu, col,
                            prefactor, u->seed, counter);
             } else {
+                const LinkMap& m = u->map;
                 for (int par = 0; par < 2; ++par)
-                    launch(ctx, name, sweep_cb2_kernel<HEATBATH>, dim3(nblocks(g.Vh, BS)), dim3(BS), 0, u->d, g, mu, par,
-                           prefactor, u->seed, counter);
+                    launch(ctx, name, sweep_cb2_kernel<HEATBATH>, dim3(map_blocks(m)), dim3(m.nthreads), 0, u->d, g, m, mu,
+                           par, prefactor, u->seed, counter);
             }
         }
     }
@@ -178,8 +176,9 @@ void sweeps(lt_field u, int n, int schedule, double prefactor, const char* name)
 
 void k_staple_field(lt_field u, lt_field out)
 {
-    launch(u->ctx, "staple_field", staple_field_kernel, dim3(nblocks(u->g.Vh, BS), 8), dim3(BS), 0,
-           (const double2*)u->d, out->d, u->g);
+    const LinkMap& m = u->map;
+    launch(u->ctx, "staple_field", staple_field_kernel, dim3(map_blocks(m), 8), dim3(m.nthreads), 0,
+           (const double2*)u->d, out->d, u->g, m);
 }
 
 void k_or_sweep(lt_field u, int n, int schedule)

@@ -42,7 +42,10 @@ def test_roundtrip_and_haar(la, lto, gpu_ctx, shape):
     flush(f"roundtrip {shape}")
     U = lto.haar(shape, 3)
     f = field(la, gpu_ctx, shape, U)
-    assert np.array_equal(f.download(), U)
+    got = f.download()
+    # rows 0 and 1 travel bit for bit; row 2 of an SU(3) link is rebuilt as conj(row0 x row1)
+    assert np.array_equal(got[..., :12], U[..., :12])
+    assert np.max(np.abs(got[..., 12:] - U[..., 12:])) <= 4e-16
     f.SetToHaar(3)
     assert rel_err(f.download(), U) <= 1e-13
     dev, bad = f.check()
