@@ -466,9 +466,8 @@ struct LinkMap {
     int nthreads; // T[0]*T[1]*T[2]*T[3] == blockDim.x
 };
 
-__device__ __forceinline__ bool map_site(const Geom& g, const LinkMap& m, int par, int c[4], int& s)
+__device__ __forceinline__ bool map_site_b(const Geom& g, const LinkMap& m, int b, int par, int c[4], int& s)
 {
-    int b = xcd_block(blockIdx.x, gridDim.x);
     int tid = threadIdx.x;
     int bz = b % m.N[3];
     b /= m.N[3];
@@ -491,6 +490,23 @@ __device__ __forceinline__ bool map_site(const Geom& g, const LinkMap& m, int pa
     c[3] = 2 * zc + flip;
     s = ((c[0] * g.L[1] + c[1]) * g.L[2] + c[2]) * g.L[3] + c[3];
     return true;
+}
+
+__device__ __forceinline__ bool map_site(const Geom& g, const LinkMap& m, int par, int c[4], int& s)
+{
+    return map_site_b(g, m, xcd_block(blockIdx.x, gridDim.x), par, c, s);
+}
+
+// Jacobi kernels over all links: the 8 (mu, parity) classes of one tile are
+// consecutive blocks of the same XCD, so the staples of all directions at a tile
+// are built from one L2-resident neighbourhood.  Grid: 8 * tiles blocks.
+__device__ __forceinline__ bool map_link_all(const Geom& g, const LinkMap& m, int& mu, int& par, int c[4], int& s)
+{
+    int b = xcd_block(blockIdx.x, gridDim.x);
+    int mp = b & 7;
+    mu = mp >> 1;
+    par = mp & 1;
+    return map_site_b(g, m, b >> 3, par, c, s);
 }
 
 // Site of the cb-th entry of a parity: s = 2*cb or 2*cb+1 with the right parity.

@@ -85,59 +85,59 @@ __device__ __forceinline__ void block_sum3(double v0, double v1, double v2, doub
     }
 }
 
-// grid (chunks per timeslice, Nt): partial sums of plaquette, raw q, energy
+template <int A0, int A1, int B0, int B1>
+__device__ __forceinline__ void pair_terms(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int par,
+                                           double& p, double& q, double& e)
+{
+    double pl;
+    M3 Fa = ah2(clover_plane(U, g, s, c, par, A0, A1, pl));
+    p += pl;
+    e += energy_term(Fa);
+    M3 Fb = ah2(clover_plane(U, g, s, c, par, B0, B1, pl));
+    p += pl;
+    e += energy_term(Fb);
+    q += retr_mul(Fa, Fb);
+}
+
+// grid (chunks per timeslice, Nt, 3 plane pairs): partial sums of the plaquette,
+// raw q and energy.  The pairs (01,23), (02,13), (03,12) enter
+// q = tr F01F23 - tr F02F13 + tr F03F12; one pair per thread keeps registers low.
 __global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, Geom g, int Vs, double* part)
 {
     const int t = blockIdx.y;
     const int r = blockIdx.x * BS + threadIdx.x;
+    const int pair = blockIdx.z;
     double p = 0.0, q = 0.0, e = 0.0;
     if (r < Vs) {
         const int s = t * Vs + r;
         int c[4];
         coords_of(g, s, c);
         const int par = parity_of(c);
-        double pl;
-        // pairs (01,23), (02,13), (03,12) for q = tr F01F23 - tr F02F13 + tr F03F12
-        {
-            M3 Fa = ah2(clover_plane(U, g, s, c, par, 0, 1, pl));
-            p += pl;
-            M3 Fb = ah2(clover_plane(U, g, s, c, par, 2, 3, pl));
-            p += pl;
-            q += retr_mul(Fa, Fb);
-            e += energy_term(Fa) + energy_term(Fb);
-        }
-        {
-            M3 Fa = ah2(clover_plane(U, g, s, c, par, 0, 2, pl));
-            p += pl;
-            M3 Fb = ah2(clover_plane(U, g, s, c, par, 1, 3, pl));
-            p += pl;
-            q -= retr_mul(Fa, Fb);
-            e += energy_term(Fa) + energy_term(Fb);
-        }
-        {
-            M3 Fa = ah2(clover_plane(U, g, s, c, par, 0, 3, pl));
-            p += pl;
-            M3 Fb = ah2(clover_plane(U, g, s, c, par, 1, 2, pl));
-            p += pl;
-            q += retr_mul(Fa, Fb);
-            e += energy_term(Fa) + energy_term(Fb);
+        if (pair == 0) {
+            pair_terms<0, 1, 2, 3>(U, g, s, c, par, p, q, e);
+        } else if (pair == 1) {
+            pair_terms<0, 2, 1, 3>(U, g, s, c, par, p, q, e);
+            q = -q;
+        } else {
+            pair_terms<0, 3, 1, 2>(U, g, s, c, par, p, q, e);
         }
     }
-    block_sum3(p, q, e, part + 3 * ((size_t)t * gridDim.x + blockIdx.x));
+    block_sum3(p, q, e, part + 3 * (((size_t)pair * gridDim.y + t) * gridDim.x + blockIdx.x));
 }
 
-// one thread per timeslice: ordered sum over chunks
+// one thread per timeslice: ordered sum over pairs and chunks
 __global__ void obs_finish_kernel(const double* part, int nchunk, int nt, double* out)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int k = 0; k < nchunk; ++k) {
-        const double* p = part + 3 * ((size_t)t * nchunk + k);
-        a += p[0];
-        b += p[1];
-        c += p[2];
-    }
+    for (int pr = 0; pr < 3; ++pr)
+        for (int k = 0; k < nchunk; ++k) {
+            const double* p = part + 3 * (((size_t)pr * nt + t) * nchunk + k);
+            a += p[0];
+            b += p[1];
+            c += p[2];
+        }
     out[3 * t] = a;
     out[3 * t + 1] = b;
     out[3 * t + 2] = c;
@@ -152,9 +152,9 @@ void k_observables(lt_field u, double* host_out)
     const int nt = g.L[0];
     const int Vs = g.V / nt;
     const unsigned nchunk = nblocks(Vs, BS);
-    double* part = ctx->scratch_for(3 * (size_t)nchunk * nt + 3 * (size_t)nt);
-    double* fin = part + 3 * (size_t)nchunk * nt;
-    launch(ctx, "observables", obs_kernel, dim3(nchunk, nt), dim3(BS), 0, (const double2*)u->d, g, Vs, part);
+    double* part = ctx->scratch_for(9 * (size_t)nchunk * nt + 3 * (size_t)nt);
+    double* fin = part + 9 * (size_t)nchunk * nt;
+    launch(ctx, "observables", obs_kernel, dim3(nchunk, nt, 3), dim3(BS), 0, (const double2*)u->d, g, Vs, part);
     launch(ctx, "observables_finish", obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, (const double*)part,
            (int)nchunk, nt, fin);
     LT_HIP_CHECK(hipMemcpyAsync(host_out, fin, 3 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -15,9 +15,8 @@ namespace {
 __global__ void LT_LINK_KERNEL stout_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout, Geom g,
                                             LinkMap m, double rho)
 {
-    int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
-    int c[4], s;
-    if (!map_site(g, m, par, c, s)) return;
+    int mu, par, c[4], s;
+    if (!map_link_all(g, m, mu, par, c, s)) return;
     M3 st = staple(Uin, g, s, c, mu);
     M3 L = ldu(Uin, g, s, mu, par);
     M3 A = algebra(mul_nd(st, L));   // C = 1/2 (A - A^+) - 1/6 tr(A - A^+)
@@ -30,9 +29,8 @@ __global__ void LT_LINK_KERNEL flow_stage_kernel(const double2* __restrict__ Uin
                                                  double2* __restrict__ Z, Geom g, LinkMap m, double c_old, double c_new,
                                                  double c_exp, int read_z, int write_z)
 {
-    int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
-    int c[4], s;
-    if (!map_site(g, m, par, c, s)) return;
+    int mu, par, c[4], s;
+    if (!map_link_all(g, m, mu, par, c, s)) return;
     M3 st = staple(Uin, g, s, c, mu);
     M3 L = ldu(Uin, g, s, mu, par);
     M3 F = algebra(mul_nd(st, L));
@@ -54,13 +52,13 @@ __global__ void LT_LINK_KERNEL flow_stage_kernel(const double2* __restrict__ Uin
 
 void k_stout(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double rho)
 {
-    launch(ctx, "stout", stout_kernel, dim3(map_blocks(m), 8), dim3(m.nthreads), 0, in, out, g, m, rho);
+    launch(ctx, "stout", stout_kernel, dim3(8 * map_blocks(m)), dim3(m.nthreads), 0, in, out, g, m, rho);
 }
 
 void k_flow_stage(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double2* z, double c_old,
                   double c_new, double c_exp, int read_z, int write_z)
 {
-    launch(ctx, "flow_stage", flow_stage_kernel, dim3(map_blocks(m), 8), dim3(m.nthreads), 0, in, out, z, g, m, c_old,
+    launch(ctx, "flow_stage", flow_stage_kernel, dim3(8 * map_blocks(m)), dim3(m.nthreads), 0, in, out, z, g, m, c_old,
            c_new, c_exp, read_z, write_z);
 }
 
