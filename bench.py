@@ -29,6 +29,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LINK_UPDATE = 1296.0
 BYTES_PER_LINK_STAGE = 480.0
+LAYOUT_BYTES_PER_LINK_UPDATE = 864.0   # 8 half-field reads + 1 write of 96-byte (two-row) links
+
+
+def measured_traffic(path):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by the
+    factor the same counter reads for a plain copy kernel of known size)."""
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(L, beta, budget_s):
@@ -167,9 +180,12 @@ def main():
                                "(2-colour checkerboard)",
                    "lattice": list(shape), "beta": args.beta, "schedule": "checkerboard-2"},
         "roofline": {"bound": "hbm", "achieved": or_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": or_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": or_gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", f"or_cb2_{L}x4_traffic.json")),
                      "kernel": "or_cb2 (one colour of one direction)", "avg_launch_ms": or_ms,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     # links are stored as two rows (96 B): the minimum this layout moves
+                     "layout_bytes_per_launch": LAYOUT_BYTES_PER_LINK_UPDATE * V / 2},
         "hb_kernel": {"avg_launch_ms": hb_ms, "achieved_GBs": hb_gbs},
         "hbm_GBs_per_update": BYTES_PER_LINK_UPDATE * 5 * 4 * V / (ms_per_step * 1e-3) / 1e9,
         "flow_rk3": {"ms_per_step": flow_ms, "model_GBs": flow_gbs, "stage_avg_launch_ms": stage_ms,

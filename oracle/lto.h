@@ -12,7 +12,10 @@
  * 18 doubles per link, row-major, interleaved re/im (Eigen
  * Matrix<std::complex<double>,3,3,RowMajor>, LettuceGauge/defines.hpp:122).
  *
- * Pinning status (which check pins each function) is listed in oracle/PINNING.md.
+ * What pins each function: the reference's own outputs at 16^4 (golden vectors in
+ * tests/golden/, written from the compiled reference), and for what those leave
+ * open, closed forms, invariants and brute force (tests/test_oracle_invariants.py).
+ * The Philox streams and the Haar start have no reference counterpart.
  */
 #ifndef LTO_H
 #define LTO_H

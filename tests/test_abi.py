@@ -61,3 +61,19 @@ def test_no_device_reports_error_not_crash():
     rc = lib.lt_ctx_create(0, ctypes.byref(h))
     assert rc < 0
     assert len(lib.lt_last_error()) > 0
+
+
+def test_cpp_layer_compiles_and_links(tmp_path):
+    """The header-only C++ layer builds a reference-style driver against the library."""
+    import shutil
+    import subprocess
+    from lettuce_amd import _lib
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    exe = tmp_path / "driver"
+    cmd = [gxx, "-std=c++17", "-O1", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(ROOT, "tests", "cpp", "driver_example.cpp"), "-o", str(exe),
+           f"-L{os.path.dirname(_lib.LIB_PATH)}", "-llettuce_hip", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
