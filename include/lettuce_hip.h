@@ -131,6 +131,64 @@ int lt_flow_destroy(lt_flow fl);
 int lt_flow_run(lt_flow fl, int n_steps);
 int lt_flow_resume(lt_flow fl, int n_steps);
 
+/* ---- HMC --------------------------------------------------------------------
+ * Integrators::HMC::{Leapfrog, OMF_2, OMF_4} (LettuceGauge/updates/hmc_gauge.hpp:48-75,
+ * 155-190, 241-295) */
+#define LT_INT_LEAPFROG 1
+#define LT_INT_OMF2     2
+#define LT_INT_OMF4     4
+
+/* a field of general 3x3 matrices (momenta, forces): all nine entries stored */
+int lt_matrix_field_create(lt_ctx ctx, int nt, int nx, int ny, int nz, lt_field *out);
+/* HMCKernel::RandomMomentum (hmc_gauge.hpp:400-428) with Philox Gaussians */
+int lt_hmc_random_momentum(lt_field P, uint64_t seed, uint64_t counter);
+/* HMCKernel::UpdateMomenta, P += eps*beta/6 * Algebra(st U^+) (hmc_gauge.hpp:450-464) */
+int lt_hmc_update_momenta(lt_field U, lt_field P, double eps, double beta);
+/* HMCKernel::UpdateFields, U <- exp(eps P) U, Gram-Schmidt (hmc_gauge.hpp:505-523) */
+int lt_hmc_update_fields(lt_field U, lt_field P, double eps);
+/* HMCKernel::Hamiltonian, Wilson action + kinetic -sum Re tr P^2 (hmc_gauge.hpp:525-541) */
+int lt_hmc_hamiltonian(lt_field U, lt_field P, double beta, double *H);
+/* molecular-dynamics part of a trajectory on given momenta */
+int lt_hmc_integrate(lt_field U, lt_field P, double beta, int integrator, double tau, int n_step);
+/* HMCKernel::operator() (hmc_gauge.hpp:549-583): copy, momenta from U's Philox state
+ * (one counter value per trajectory), integrate, Metropolis (restores U on rejection) */
+int lt_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, double beta, int integrator, double tau,
+                      int n_step, int metropolis, double *dH, int *accepted);
+
+/* ---- collective variable and its adjoint -----------------------------------
+ * Workspace of HMCMetaDData (hmc_metadynamics.hpp:251-260): n stout levels with
+ * their exponent tapes, the clover planes and the recursion's Lambda buffer. */
+typedef struct lt_cv_s *lt_cv;
+int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, double rho, lt_cv *out);
+int lt_cv_destroy(lt_cv cv);
+/* StoutSmearing4DWithConstants x n, CalculateClover<1>, TopChargeClover(FullTensor)
+ * (stout_smearing.hpp:92-119, clover.hpp:94-152, topological_charge.hpp:202-231) */
+int lt_cv_forward(lt_cv cv, lt_field U, double *Q);
+/* F = dvdq * dQ/dU: CloverDerivative on the last level then StoutForceRecursion down
+ * to level 0 (clover.hpp:238-321, stout_smearing.hpp:122-234); F is a matrix field */
+int lt_cv_backward(lt_cv cv, double dvdq, lt_field F);
+
+/* ---- metadynamics bias potential (MetaBiasPotential, metadynamics.hpp:29-507) ---- */
+typedef struct lt_bias_s *lt_bias;
+int lt_bias_create(double cv_min, double cv_max, int bins, double weight, double width, int well_tempered,
+                   double well_tempered_parameter, double threshold_weight, lt_bias *out);
+int lt_bias_destroy(lt_bias b);
+/* UpdatePotential / UpdatePotentialSymmetric (metadynamics.hpp:103-147) */
+int lt_bias_deposit(lt_bias b, double cv, int symmetric);
+/* ReturnPotential / ReturnDerivative (metadynamics.hpp:151-187) */
+int lt_bias_potential(lt_bias b, double cv, double *v);
+int lt_bias_derivative(lt_bias b, double cv, double *d);
+/* read (set = 0) or write (set = 1) the bins + 1 grid values / the current CV */
+int lt_bias_grid(lt_bias b, double *values, int n, int set);
+int lt_bias_cv(lt_bias b, double *cv, int set);
+
+/* HMCMetaDKernel::operator() (hmc_metadynamics.hpp:552-655): MetaD-HMC trajectory with
+ * the adjoint force dV/dQ * dQ/dU, path-sample bias updates when update_bias != 0;
+ * F is a matrix field used as force scratch */
+int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F, lt_cv cv, lt_bias bias,
+                            double beta, int integrator, double tau, int n_step, int metropolis,
+                            int update_bias, double *dH, int *accepted, double *cv_new);
+
 #ifdef __cplusplus
 }
 #endif

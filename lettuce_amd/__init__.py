@@ -297,3 +297,187 @@ class GradientFlowKernel:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# HMC, collective variable + adjoint, metadynamics (updates/hmc_gauge.hpp,
+# updates/hmc_metadynamics.hpp, actions/bias_potential/metadynamics.hpp)
+
+LEAPFROG = 1
+OMF2 = 2
+OMF4 = 4
+_INTEGRATORS = {"leapfrog": LEAPFROG, "omf2": OMF2, "omf4": OMF4}
+
+
+class MatrixField(GaugeField):
+    """Field of general 3x3 matrices (momenta, forces)."""
+
+    def __init__(self, ctx, shape):
+        self.ctx = ctx
+        self.shape = tuple(int(x) for x in shape)
+        h = ctypes.c_void_p()
+        call("lt_matrix_field_create", ctx.handle, *self.shape, ctypes.byref(h))
+        self._h = h
+
+
+def RandomMomentum(P, seed, counter=0):
+    call("lt_hmc_random_momentum", P.handle, seed, counter)
+
+
+def UpdateMomenta(U, P, eps, beta):
+    call("lt_hmc_update_momenta", U.handle, P.handle, eps, beta)
+
+
+def UpdateFields(U, P, eps):
+    call("lt_hmc_update_fields", U.handle, P.handle, eps)
+
+
+def Hamiltonian(U, P, beta):
+    h = ctypes.c_double()
+    call("lt_hmc_hamiltonian", U.handle, P.handle, beta, ctypes.byref(h))
+    return h.value
+
+
+def Integrate(U, P, beta, integrator="omf2", tau=1.0, n_step=10):
+    call("lt_hmc_integrate", U.handle, P.handle, beta, _INTEGRATORS[integrator], tau, n_step)
+
+
+class HMCKernel:
+    """GaugeUpdates::HMCKernel(U, U_copy, Momentum, Integrator, Action, prng, ...) (hmc_gauge.hpp:381-600);
+    momenta come from U's Philox state, one counter value per trajectory."""
+
+    def __init__(self, U, U_copy, P, integrator="omf2", beta=6.0, trajectory_length=1.0):
+        self.U, self.U_copy, self.P = U, U_copy, P
+        self.integrator = _INTEGRATORS[integrator]
+        self.beta = beta
+        self.trajectory_length = trajectory_length
+        self.delta_H = 0.0
+
+    def __call__(self, n_step, metropolis_step=True):
+        d = ctypes.c_double()
+        a = ctypes.c_int()
+        call("lt_hmc_trajectory", self.U.handle, self.P.handle, self.U_copy.handle, self.beta, self.integrator,
+             self.trajectory_length, n_step, 1 if metropolis_step else 0, ctypes.byref(d), ctypes.byref(a))
+        self.delta_H = d.value
+        return bool(a.value)
+
+
+class CollectiveVariable:
+    """Workspace of the smeared clover charge and its adjoint (HMCMetaDData, hmc_metadynamics.hpp:251-260)."""
+
+    def __init__(self, ctx, shape, n_levels=4, rho=0.12):
+        h = ctypes.c_void_p()
+        call("lt_cv_create", ctx.handle, *shape, n_levels, rho, ctypes.byref(h))
+        self._h = h
+        self.shape = tuple(shape)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def forward(self, U):
+        q = ctypes.c_double()
+        call("lt_cv_forward", self._h, U.handle, ctypes.byref(q))
+        return q.value
+
+    def backward(self, dvdq, F):
+        call("lt_cv_backward", self._h, dvdq, F.handle)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            call("lt_cv_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MetaBiasPotential:
+    """MetaBiasPotential(CV_min, CV_max, bin_number, weight, width, well_tempered_parameter, threshold_weight)
+    (metadynamics.hpp:58-91); well_tempered selects the update rule (a compile-time switch in the reference)."""
+
+    def __init__(self, cv_min, cv_max, bins, weight, width, well_tempered_parameter=1.0, threshold_weight=1000.0,
+                 well_tempered=False):
+        h = ctypes.c_void_p()
+        call("lt_bias_create", cv_min, cv_max, bins, weight, width, 1 if well_tempered else 0,
+             well_tempered_parameter, threshold_weight, ctypes.byref(h))
+        self._h = h
+        self.bins = bins
+
+    @property
+    def handle(self):
+        return self._h
+
+    def UpdatePotential(self, cv):
+        call("lt_bias_deposit", self._h, cv, 0)
+
+    def UpdatePotentialSymmetric(self, cv):
+        call("lt_bias_deposit", self._h, cv, 1)
+
+    def ReturnPotential(self, cv):
+        v = ctypes.c_double()
+        call("lt_bias_potential", self._h, cv, ctypes.byref(v))
+        return v.value
+
+    def ReturnDerivative(self, cv):
+        v = ctypes.c_double()
+        call("lt_bias_derivative", self._h, cv, ctypes.byref(v))
+        return v.value
+
+    def grid(self):
+        a = np.zeros(self.bins + 1)
+        call("lt_bias_grid", self._h, _dp(a), self.bins + 1, 0)
+        return a
+
+    def set_grid(self, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        call("lt_bias_grid", self._h, _dp(a), self.bins + 1, 1)
+
+    def ReturnCV_current(self):
+        v = ctypes.c_double()
+        call("lt_bias_cv", self._h, ctypes.byref(v), 0)
+        return v.value
+
+    def SetCV_current(self, cv):
+        v = ctypes.c_double(cv)
+        call("lt_bias_cv", self._h, ctypes.byref(v), 1)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            call("lt_bias_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HMCMetaDKernel:
+    """GaugeUpdates::HMCMetaDKernel (hmc_metadynamics.hpp:264-675) with the adjoint force."""
+
+    def __init__(self, U, U_copy, P, F, bias, cv, integrator="omf2", beta=6.0, trajectory_length=1.0,
+                 update_bias=True):
+        self.U, self.U_copy, self.P, self.F, self.bias, self.cv = U, U_copy, P, F, bias, cv
+        self.integrator = _INTEGRATORS[integrator]
+        self.beta = beta
+        self.trajectory_length = trajectory_length
+        self.update_bias = update_bias
+        self.delta_H = 0.0
+        self.cv_value = 0.0
+
+    def __call__(self, n_step, metropolis_step=True):
+        d = ctypes.c_double()
+        a = ctypes.c_int()
+        q = ctypes.c_double()
+        call("lt_metad_hmc_trajectory", self.U.handle, self.P.handle, self.U_copy.handle, self.F.handle,
+             self.cv.handle, self.bias.handle, self.beta, self.integrator, self.trajectory_length, n_step,
+             1 if metropolis_step else 0, 1 if self.update_bias else 0, ctypes.byref(d), ctypes.byref(a),
+             ctypes.byref(q))
+        self.delta_H = d.value
+        self.cv_value = q.value
+        return bool(a.value)

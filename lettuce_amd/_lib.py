@@ -52,6 +52,28 @@ SIGNATURES = {
     "lt_flow_destroy": [c_flow],
     "lt_flow_run": [c_flow, I],
     "lt_flow_resume": [c_flow, I],
+    "lt_matrix_field_create": [c_ctx, I, I, I, I, ctypes.POINTER(c_field)],
+    "lt_hmc_random_momentum": [c_field, ctypes.c_uint64, ctypes.c_uint64],
+    "lt_hmc_update_momenta": [c_field, c_field, ctypes.c_double, ctypes.c_double],
+    "lt_hmc_update_fields": [c_field, c_field, ctypes.c_double],
+    "lt_hmc_hamiltonian": [c_field, c_field, ctypes.c_double, D],
+    "lt_hmc_integrate": [c_field, c_field, ctypes.c_double, I, ctypes.c_double, I],
+    "lt_hmc_trajectory": [c_field, c_field, c_field, ctypes.c_double, I, ctypes.c_double, I, I, D,
+                          ctypes.POINTER(I)],
+    "lt_cv_create": [c_ctx, I, I, I, I, I, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)],
+    "lt_cv_destroy": [ctypes.c_void_p],
+    "lt_cv_forward": [ctypes.c_void_p, c_field, D],
+    "lt_cv_backward": [ctypes.c_void_p, ctypes.c_double, c_field],
+    "lt_bias_create": [ctypes.c_double, ctypes.c_double, I, ctypes.c_double, ctypes.c_double, I, ctypes.c_double,
+                       ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)],
+    "lt_bias_destroy": [ctypes.c_void_p],
+    "lt_bias_deposit": [ctypes.c_void_p, ctypes.c_double, I],
+    "lt_bias_potential": [ctypes.c_void_p, ctypes.c_double, D],
+    "lt_bias_derivative": [ctypes.c_void_p, ctypes.c_double, D],
+    "lt_bias_grid": [ctypes.c_void_p, D, I, I],
+    "lt_bias_cv": [ctypes.c_void_p, D, I],
+    "lt_metad_hmc_trajectory": [c_field, c_field, c_field, c_field, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                I, ctypes.c_double, I, I, I, D, ctypes.POINTER(I), D],
 }
 
 

@@ -328,7 +328,6 @@ int lt_field_upload(lt_field f, const double* aos)
         if (e == hipSuccess) {
             try {
                 lt::k_pack(f, stage);
-                f->su3 = true;
             } catch (...) {
                 (void)hipStreamSynchronize(f->ctx->stream);
                 (void)hipFree(stage);

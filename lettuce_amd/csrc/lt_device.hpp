@@ -550,4 +550,89 @@ __device__ __forceinline__ M3 staple(const double2* __restrict__ U, const Geom& 
     return st;
 }
 
+// Sum of the four clover leaves of plane (mu, nu) at site s (leaf orientation of
+// the reference's CalculateClover<1,1>); leaf I (the plaquette) is returned too.
+__device__ __forceinline__ M3 clover_plane(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int par,
+                                           int mu, int nu, double& retr_plaq)
+{
+    const int q = par ^ 1;
+    const int smu = up(g, s, c[mu], mu), snu = up(g, s, c[nu], nu);
+    const int smd = dn(g, s, c[mu], mu), snd = dn(g, s, c[nu], nu);
+    const int smd_nu = up(g, smd, c[nu], nu), smd_nd = dn(g, smd, c[nu], nu), smu_nd = up(g, snd, c[mu], mu);
+    const M3 Umu = ldu(U, g, s, mu, par), Unu = ldu(U, g, s, nu, par);
+    // leaf I: U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+    M3 l = mul_nd(mul_nd(mul(Umu, ldu(U, g, smu, nu, q)), ldu(U, g, snu, mu, q)), Unu);
+    retr_plaq = l.m[0].re + l.m[4].re + l.m[8].re;
+    M3 C = l;
+    // leaf II: U_nu(x) U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
+    const M3 Umu_md = ldu(U, g, smd, mu, q);
+    l = mul(mul_nd(mul_nd(Unu, ldu(U, g, smd_nu, mu, par)), ldu(U, g, smd, nu, q)), Umu_md);
+    acc(C, l);
+    // leaf III: U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu)
+    l = mul(mul(mul_nd(dag(Umu_md), ldu(U, g, smd_nd, nu, par)), ldu(U, g, smd_nd, mu, par)), ldu(U, g, snd, nu, q));
+    acc(C, l);
+    // leaf IV: U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^+
+    l = mul_nd(mul(mul_dn(ldu(U, g, snd, nu, q), ldu(U, g, snd, mu, q)), ldu(U, g, smu_nd, nu, par)), Umu);
+    acc(C, l);
+    return C;
+}
+
+
+// Derivative constants of the Cayley-Hamilton exponential (Morningstar-Peardon
+// r_ij, b_ij, B_1, B_2 with the sign-flip conjugation rules); returns B1, B2 and
+// f1, f2 for the stout-force Lambda.
+__device__ __forceinline__ void exp_deriv(const ExpC& e, const M3& Q, const M3& Q2, M3& B1, M3& B2, cd& f1, cd& f2)
+{
+    const double u = e.u, u2 = e.u2, w2 = e.w2;
+    const double xi0 = e.xi0, xi1 = xi1_of(e.w), cw = e.cosw;
+    const cd em = e.exp_miu, e2 = e.exp_2iu;
+    const cd ix = cd{0.0, xi0};
+    const cd I = cd{0.0, 1.0};
+    const cd f0 = e.denom * e.h0;
+    f1 = e.denom * e.h1;
+    f2 = e.denom * e.h2;
+    const cd r10 = 2.0 * (cd{u, u2 - w2} * e2)
+                 + 2.0 * (em * ((4.0 * u) * cd{2.0, -u} * cw + ix * cd{9.0 * u2 + w2, -u * (3.0 * u2 + w2)}));
+    const cd r11 = 2.0 * (cd{1.0, 2.0 * u} * e2) + em * (-2.0 * cd{1.0, -u} * cw + ix * cd{6.0 * u, w2 - 3.0 * u2});
+    const cd r12 = 2.0 * (I * e2) + I * (em * (cd{cw, 0.0} - 3.0 * cd{1.0, -u} * xi0));
+    const cd r20 = -2.0 * e2 + cd{0.0, 2.0 * u} * (em * (cd{cw, 0.0} + cd{1.0, 4.0 * u} * xi0 + cd{3.0 * u2 * xi1, 0.0}));
+    const cd r21 = -(I * (em * (cd{cw, 0.0} + cd{1.0, 2.0 * u} * xi0 - cd{3.0 * u2 * xi1, 0.0})));
+    const cd r22 = em * cd{xi0, -3.0 * u * xi1};
+    cd b10, b11, b12, b20, b21, b22;
+    if (e.c0max < 2.2250738585072014e-308) {
+        b10 = b11 = b12 = b20 = b21 = b22 = cd{0.0, 0.0};
+    } else {
+        double bd = 9.0 * u2 - w2;
+        bd *= 2.0 * bd;
+        bd = 1.0 / bd;
+        const double p = 3.0 * u2 - w2, q = 15.0 * u2 + w2;
+        if (e.signflip) {
+            b10 = bd * conj((2.0 * u) * r10 + p * r20 - (2.0 * q) * conj(f0));
+            b11 = -bd * conj((2.0 * u) * r11 + p * r21 + (2.0 * q) * conj(f1));
+            b12 = bd * conj((2.0 * u) * r12 + p * r22 - (2.0 * q) * conj(f2));
+            b20 = -bd * conj(r10 - (3.0 * u) * r20 - (24.0 * u) * conj(f0));
+            b21 = bd * conj(r11 - (3.0 * u) * r21 + (24.0 * u) * conj(f1));
+            b22 = -bd * conj(r12 - (3.0 * u) * r22 - (24.0 * u) * conj(f2));
+        } else {
+            b10 = bd * ((2.0 * u) * r10 + p * r20 - (2.0 * q) * f0);
+            b11 = bd * ((2.0 * u) * r11 + p * r21 - (2.0 * q) * f1);
+            b12 = bd * ((2.0 * u) * r12 + p * r22 - (2.0 * q) * f2);
+            b20 = bd * (r10 - (3.0 * u) * r20 - (24.0 * u) * f0);
+            b21 = bd * (r11 - (3.0 * u) * r21 - (24.0 * u) * f1);
+            b22 = bd * (r12 - (3.0 * u) * r22 - (24.0 * u) * f2);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        B1.m[k] = b11 * Q.m[k] + b12 * Q2.m[k];
+        B2.m[k] = b21 * Q.m[k] + b22 * Q2.m[k];
+    }
+    B1.m[0] += b10;
+    B1.m[4] += b10;
+    B1.m[8] += b10;
+    B2.m[0] += b20;
+    B2.m[4] += b20;
+    B2.m[8] += b20;
+}
+
 }  // namespace lt

@@ -1,0 +1,533 @@
+// lt_hmc_api.cpp -- C ABI of the HMC chain: momenta, integrators (leapfrog, OMF2,
+// OMF4), the Hamiltonian and Metropolis step, the collective-variable forward /
+// adjoint pair, the metadynamics bias potential and the MetaD-HMC trajectory.
+// All lattice work runs in HIP kernels; only scalars (H, Q, dV/dQ) touch the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "lt_internal.hpp"
+
+namespace lt {
+void k_random_momentum(lt_field P, uint64_t seed, uint64_t counter);
+void k_force_update(lt_field U, lt_field P, const double2* F, double a, double b);
+void k_field_update(lt_field U, lt_field P, double eps);
+double k_plaquette_sum(lt_field U);
+double k_kinetic(lt_field P);
+void k_stout_tape(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double2* tape,
+                  double rho);
+double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C);
+void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
+                    double scale_factor);
+void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,
+                       double2* Sig, const double2* T, double2* Lam, double rho);
+}  // namespace lt
+
+// Collective-variable workspace: stout levels with their exponent tapes, the six
+// clover planes of the most smeared field and the Lambda buffer of the recursion.
+struct lt_cv_s {
+    lt_ctx ctx = nullptr;
+    lt::Geom g{};
+    lt::LinkMap map{};
+    int n_levels = 0;
+    double rho = 0.0;
+    std::vector<double2*> levels;   // n+1 link buffers (levels[0] = copy of the input)
+    std::vector<double2*> tapes;    // n exponent buffers
+    double2* clover = nullptr;      // 6 planes per site
+    double2* lambda = nullptr;
+    bool forward_done = false;
+};
+
+// MetaBiasPotential (LettuceGauge/actions/bias_potential/metadynamics.hpp:29-507)
+struct lt_bias_s {
+    std::vector<double> grid;
+    double cv_current = 0.0, cv_min = 0.0, cv_max = 0.0, bin_width = 0.0, bin_width_inverse = 0.0;
+    double weight = 0.0, width = 0.0, wt_param = 0.0, threshold = 0.0;
+    int bins = 0;
+    int well_tempered = 0;
+    unsigned long long exceeded = 0;
+
+    int bin_of(double cv) const { return (int)std::floor((cv - cv_min) * bin_width_inverse); }
+    double cv_of(int i) const { return cv_min + i * bin_width; }
+    // UpdatePotential (metadynamics.hpp:103-124)
+    void deposit(double cv)
+    {
+        if (cv < cv_min || cv >= cv_max) exceeded += 1;
+        const double wis = 1.0 / (width * width);
+        for (size_t b = 0; b < grid.size(); ++b) {
+            const double dist = cv - cv_of((int)b);
+            if (well_tempered)
+                grid[b] += weight * std::exp(-0.5 * wis * dist * dist - grid[b] / wt_param);
+            else
+                grid[b] += weight * std::exp(-0.5 * wis * dist * dist);
+        }
+    }
+    // ReturnPotential (metadynamics.hpp:151-179)
+    double potential(double cv) const
+    {
+        const int i = bin_of(cv);
+        if ((unsigned)i < (unsigned)(bins)) {
+            const double ic = (cv - cv_of(i)) * bin_width_inverse;
+            return grid[i] * (1.0 - ic) + ic * grid[i + 1];
+        }
+        if (i < 0) {
+            const double d = cv - cv_min;
+            return grid.front() + threshold * d * d;
+        }
+        const double d = cv - cv_max;
+        return grid.back() + threshold * d * d;
+    }
+    // ReturnDerivative (metadynamics.hpp:181-187)
+    double derivative(double cv) const
+    {
+        const double h = 0.5 * bin_width;
+        return bin_width_inverse * (potential(cv + h) - potential(cv - h));
+    }
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const lt::LtError& e) {
+        lt::set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        lt::set_error(e.what());
+        return LT_ESTATE;
+    }
+}
+
+int fail(int code, const std::string& msg)
+{
+    lt::set_error(msg);
+    return code;
+}
+
+bool same_shape(const lt::Geom& a, const lt::Geom& b)
+{
+    for (int i = 0; i < 4; ++i)
+        if (a.L[i] != b.L[i]) return false;
+    return true;
+}
+
+double2* alloc_buf(size_t n_double2)
+{
+    double2* p = nullptr;
+    if (hipMalloc(&p, n_double2 * sizeof(double2)) != hipSuccess)
+        throw lt::LtError{LT_ENOMEM, "device allocation failed"};
+    return p;
+}
+
+// Philox draw for scalar accept/reject decisions (stream 3), identical on host and device
+double host_uniform(uint64_t seed, uint64_t counter)
+{
+    uint32_t c[4] = {0xFFFFFFFFu, 3u << 24, (uint32_t)counter, (uint32_t)(counter >> 32)};
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = (uint32_t)p1;
+        c[2] = n2;
+        c[3] = (uint32_t)p0;
+    }
+    return ((double)(c[0] >> 5) * 67108864.0 + (double)(c[1] >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Integrators::HMC::{Leapfrog, OMF_2, OMF_4} (LettuceGauge/updates/hmc_gauge.hpp:48-75,
+// 155-190, 241-295); momentum updates carry the "submeasurement" flag the MetaD
+// path update uses.
+void integrate(int kind, double tau, int n_step, const std::function<void(double, bool)>& P,
+               const std::function<void(double)>& U)
+{
+    if (n_step <= 0) return;
+    const double eps = tau / n_step;
+    if (kind == LT_INT_LEAPFROG) {
+        P(0.5 * eps, false);
+        for (int k = 0; k < n_step - 1; ++k) {
+            U(eps);
+            P(eps, true);
+        }
+        U(eps);
+        P(0.5 * eps, true);
+    } else if (kind == LT_INT_OMF2) {
+        const double lam = 0.1931833275037836;
+        const double alpha = lam * eps, beta = 0.5 * eps, gamma = (1.0 - 2.0 * lam) * eps;
+        P(alpha, false);
+        U(beta);
+        P(gamma, false);
+        U(beta);
+        for (int k = 0; k < n_step - 1; ++k) {
+            P(2.0 * alpha, true);
+            U(beta);
+            P(gamma, false);
+            U(beta);
+        }
+        P(alpha, true);
+    } else {
+        const double alpha = 0.08398315262876693 * eps, beta = 0.2539785108410595 * eps;
+        const double gamma = 0.6822365335719091 * eps, delta = -0.03230286765269967 * eps;
+        const double mu = (0.5 - 0.6822365335719091 - 0.08398315262876693) * eps;
+        const double nu = (1.0 - 2.0 * -0.03230286765269967 - 2.0 * 0.2539785108410595) * eps;
+        auto body = [&]() {
+            U(beta);
+            P(gamma, false);
+            U(delta);
+            P(mu, false);
+            U(nu);
+            P(mu, false);
+            U(delta);
+            P(gamma, false);
+            U(beta);
+        };
+        P(alpha, false);
+        body();
+        for (int k = 0; k < n_step - 1; ++k) {
+            P(2.0 * alpha, true);
+            body();
+        }
+        P(alpha, true);
+    }
+}
+
+bool valid_integrator(int k) { return k == LT_INT_LEAPFROG || k == LT_INT_OMF2 || k == LT_INT_OMF4; }
+
+double hamiltonian(lt_field U, lt_field P, double beta)
+{
+    const double S = beta * (6.0 * (double)U->g.V - lt::k_plaquette_sum(U) / 3.0);
+    return S + lt::k_kinetic(P);
+}
+
+void copy_field(lt_field dst, lt_field src)
+{
+    lt::k_copy(dst->ctx, src->d, dst->d, (size_t)72 * dst->g.Vh);
+    dst->su3 = src->su3;
+}
+
+// CV forward: levels with tapes, clover planes, Q (FullTensor route, topological_charge.hpp:202-231)
+double cv_forward(lt_cv cv, lt_field U)
+{
+    lt::k_copy(cv->ctx, U->d, cv->levels[0], (size_t)72 * cv->g.Vh);
+    for (int k = 0; k < cv->n_levels; ++k)
+        lt::k_stout_tape(cv->ctx, cv->g, cv->map, cv->levels[k], cv->levels[k + 1], cv->tapes[k], cv->rho);
+    double Q = lt::k_clover_charge(cv->ctx, cv->g, cv->levels[cv->n_levels], cv->clover);
+    cv->forward_done = true;
+    return Q;
+}
+
+// CV adjoint into F: dvdq * dQ/dU through the clover derivative and n recursions
+void cv_backward(lt_cv cv, double dvdq, double2* F)
+{
+    const int n = cv->n_levels;
+    lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[n], cv->clover, F, dvdq);
+    for (int k = n; k > 0; --k)
+        lt::k_stout_recursion(cv->ctx, cv->g, cv->map, cv->levels[k - 1], cv->levels[k], F, cv->tapes[k - 1],
+                              cv->lambda, cv->rho);
+}
+
+// MetaCharge (hmc_metadynamics.hpp:302-310): n stout levels, clover Q
+double cv_charge(lt_cv cv, lt_field U)
+{
+    return cv_forward(cv, U);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lt_matrix_field_create(lt_ctx ctx, int nt, int nx, int ny, int nz, lt_field* out)
+{
+    int rc = lt_field_create(ctx, nt, nx, ny, nz, out);
+    if (rc == LT_OK) (*out)->su3 = false;
+    return rc;
+}
+
+int lt_hmc_random_momentum(lt_field P, uint64_t seed, uint64_t counter)
+{
+    if (!P) return fail(LT_EINVAL, "null field");
+    return guarded([&] {
+        lt::k_random_momentum(P, seed, counter);
+        P->su3 = false;
+        return LT_OK;
+    });
+}
+
+int lt_hmc_update_momenta(lt_field U, lt_field P, double eps, double beta)
+{
+    if (!U || !P || !same_shape(U->g, P->g)) return fail(LT_EINVAL, "lt_hmc_update_momenta: bad fields");
+    return guarded([&] {
+        lt::k_force_update(U, P, nullptr, eps * beta / 6.0, 0.0);
+        return LT_OK;
+    });
+}
+
+int lt_hmc_update_fields(lt_field U, lt_field P, double eps)
+{
+    if (!U || !P || !same_shape(U->g, P->g)) return fail(LT_EINVAL, "lt_hmc_update_fields: bad fields");
+    return guarded([&] {
+        lt::k_field_update(U, P, eps);
+        return LT_OK;
+    });
+}
+
+int lt_hmc_hamiltonian(lt_field U, lt_field P, double beta, double* H)
+{
+    if (!U || !P || !H || !same_shape(U->g, P->g)) return fail(LT_EINVAL, "lt_hmc_hamiltonian: bad arguments");
+    return guarded([&] {
+        *H = hamiltonian(U, P, beta);
+        return LT_OK;
+    });
+}
+
+int lt_hmc_integrate(lt_field U, lt_field P, double beta, int integrator, double tau, int n_step)
+{
+    if (!U || !P || !same_shape(U->g, P->g) || !valid_integrator(integrator) || n_step < 0)
+        return fail(LT_EINVAL, "lt_hmc_integrate: bad arguments");
+    return guarded([&] {
+        integrate(
+            integrator, tau, n_step, [&](double e, bool) { lt::k_force_update(U, P, nullptr, e * beta / 6.0, 0.0); },
+            [&](double e) { lt::k_field_update(U, P, e); });
+        return LT_OK;
+    });
+}
+
+int lt_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, double beta, int integrator, double tau, int n_step,
+                      int metropolis, double* dH, int* accepted)
+{
+    if (!U || !P || !U_copy || !same_shape(U->g, P->g) || !same_shape(U->g, U_copy->g) || U == U_copy ||
+        !valid_integrator(integrator) || n_step < 0)
+        return fail(LT_EINVAL, "lt_hmc_trajectory: bad arguments");
+    return guarded([&] {
+        // HMCKernel::operator() (hmc_gauge.hpp:549-583)
+        const uint64_t counter = U->counter++;
+        copy_field(U_copy, U);
+        lt::k_random_momentum(P, U->seed, counter);
+        P->su3 = false;
+        const double H0 = hamiltonian(U, P, beta);
+        integrate(
+            integrator, tau, n_step, [&](double e, bool) { lt::k_force_update(U, P, nullptr, e * beta / 6.0, 0.0); },
+            [&](double e) { lt::k_field_update(U, P, e); });
+        const double H1 = hamiltonian(U, P, beta);
+        const double d = H1 - H0;
+        int acc = 1;
+        if (metropolis) {
+            const double q = host_uniform(U->seed, counter);
+            acc = q <= std::exp(-d);
+            if (!acc) copy_field(U, U_copy);
+        }
+        if (dH) *dH = d;
+        if (accepted) *accepted = acc;
+        return LT_OK;
+    });
+}
+
+int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, double rho, lt_cv* out)
+{
+    if (!ctx || !out || n_levels < 0 || n_levels > 64) return fail(LT_EINVAL, "lt_cv_create: bad arguments");
+    if (nt < 2 || nx < 2 || ny < 2 || nz < 2 || (nt | nx | ny | nz) & 1)
+        return fail(LT_EINVAL, "lt_cv_create: extents must be even and >= 2");
+    return guarded([&] {
+        LT_HIP_CHECK(hipSetDevice(ctx->device));
+        auto* cv = new lt_cv_s();
+        cv->ctx = ctx;
+        cv->g = lt::make_geom(nt, nx, ny, nz);
+        cv->map = lt::default_linkmap(cv->g);
+        cv->n_levels = n_levels;
+        cv->rho = rho;
+        const size_t nl = (size_t)72 * cv->g.Vh;
+        try {
+            for (int k = 0; k <= n_levels; ++k) cv->levels.push_back(alloc_buf(nl));
+            for (int k = 0; k < n_levels; ++k) cv->tapes.push_back(alloc_buf(nl));
+            cv->clover = alloc_buf((size_t)108 * cv->g.Vh);
+            cv->lambda = alloc_buf(nl);
+        } catch (...) {
+            for (auto p : cv->levels) (void)hipFree(p);
+            for (auto p : cv->tapes) (void)hipFree(p);
+            if (cv->clover) (void)hipFree(cv->clover);
+            delete cv;
+            throw;
+        }
+        *out = cv;
+        return LT_OK;
+    });
+}
+
+int lt_cv_destroy(lt_cv cv)
+{
+    if (!cv) return LT_OK;
+    return guarded([&] {
+        LT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+        for (auto p : cv->levels) (void)hipFree(p);
+        for (auto p : cv->tapes) (void)hipFree(p);
+        (void)hipFree(cv->clover);
+        (void)hipFree(cv->lambda);
+        delete cv;
+        return LT_OK;
+    });
+}
+
+int lt_cv_forward(lt_cv cv, lt_field U, double* Q)
+{
+    if (!cv || !U || !Q || !same_shape(cv->g, U->g) || cv->ctx != U->ctx)
+        return fail(LT_EINVAL, "lt_cv_forward: bad arguments");
+    return guarded([&] {
+        *Q = cv_forward(cv, U);
+        return LT_OK;
+    });
+}
+
+int lt_cv_backward(lt_cv cv, double dvdq, lt_field F)
+{
+    if (!cv || !F || !same_shape(cv->g, F->g) || cv->ctx != F->ctx) return fail(LT_EINVAL, "lt_cv_backward: bad arguments");
+    if (!cv->forward_done) return fail(LT_ESTATE, "lt_cv_backward: call lt_cv_forward first");
+    return guarded([&] {
+        cv_backward(cv, dvdq, F->d);
+        F->su3 = false;
+        return LT_OK;
+    });
+}
+
+int lt_bias_create(double cv_min, double cv_max, int bins, double weight, double width, int well_tempered,
+                   double well_tempered_parameter, double threshold_weight, lt_bias* out)
+{
+    if (!out) return fail(LT_EINVAL, "lt_bias_create: null out");
+    // MetaBiasPotential constructor checks (metadynamics.hpp:70-77)
+    if (!(cv_min < cv_max)) return fail(LT_EINVAL, "CV_min < CV_max required");
+    if (bins <= 0) return fail(LT_EINVAL, "bin_number > 0 required");
+    auto* b = new lt_bias_s();
+    b->cv_min = cv_min;
+    b->cv_max = cv_max;
+    b->bins = bins;
+    b->bin_width = (cv_max - cv_min) / bins;
+    b->bin_width_inverse = 1.0 / b->bin_width;
+    b->weight = weight;
+    b->width = width;
+    b->well_tempered = well_tempered;
+    b->wt_param = well_tempered_parameter;
+    b->threshold = threshold_weight;
+    b->grid.assign((size_t)bins + 1, 0.0);
+    *out = b;
+    return LT_OK;
+}
+
+int lt_bias_destroy(lt_bias b)
+{
+    delete b;
+    return LT_OK;
+}
+
+int lt_bias_deposit(lt_bias b, double cv, int symmetric)
+{
+    if (!b) return fail(LT_EINVAL, "null bias");
+    b->deposit(cv);
+    if (symmetric) b->deposit(-cv);
+    return LT_OK;
+}
+
+int lt_bias_potential(lt_bias b, double cv, double* v)
+{
+    if (!b || !v) return fail(LT_EINVAL, "null argument");
+    *v = b->potential(cv);
+    return LT_OK;
+}
+
+int lt_bias_derivative(lt_bias b, double cv, double* d)
+{
+    if (!b || !d) return fail(LT_EINVAL, "null argument");
+    *d = b->derivative(cv);
+    return LT_OK;
+}
+
+int lt_bias_grid(lt_bias b, double* values, int n, int set)
+{
+    if (!b || !values || n != (int)b->grid.size()) return fail(LT_EINVAL, "lt_bias_grid: size must be bins + 1");
+    if (set)
+        std::memcpy(b->grid.data(), values, sizeof(double) * n);
+    else
+        std::memcpy(values, b->grid.data(), sizeof(double) * n);
+    return LT_OK;
+}
+
+int lt_bias_cv(lt_bias b, double* cv, int set)
+{
+    if (!b || !cv) return fail(LT_EINVAL, "null argument");
+    if (set)
+        b->cv_current = *cv;
+    else
+        *cv = b->cv_current;
+    return LT_OK;
+}
+
+// HMCMetaDKernel::operator() (hmc_metadynamics.hpp:552-655) with path updates:
+// each "submeasurement" momentum update records the CV it used; after the
+// Metropolis step the samples (or CV_old repeated, if rejected) are deposited
+// symmetrically when update_bias is set.
+int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F, lt_cv cv, lt_bias bias, double beta,
+                            int integrator, double tau, int n_step, int metropolis, int update_bias, double* dH,
+                            int* accepted, double* cv_new)
+{
+    if (!U || !P || !U_copy || !F || !cv || !bias || !valid_integrator(integrator) || n_step < 0 || U == U_copy ||
+        !same_shape(U->g, P->g) || !same_shape(U->g, U_copy->g) || !same_shape(U->g, F->g) || !same_shape(U->g, cv->g))
+        return fail(LT_EINVAL, "lt_metad_hmc_trajectory: bad arguments");
+    return guarded([&] {
+        const uint64_t counter = U->counter++;
+        copy_field(U_copy, U);
+        lt::k_random_momentum(P, U->seed, counter);
+        P->su3 = false;
+        F->su3 = false;
+        const double cv_old = cv_charge(cv, U);
+        const double H0 = hamiltonian(U, P, beta) + bias->potential(cv_old);
+        std::vector<double> samples;
+        samples.reserve(n_step);
+        integrate(
+            integrator, tau, n_step,
+            [&](double e, bool sub) {
+                // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436) + UpdateMomenta (:441-459)
+                const double q = cv_forward(cv, U);
+                if (sub) samples.push_back(q);
+                cv_backward(cv, bias->derivative(q), F->d);
+                lt::k_force_update(U, P, F->d, e * beta / 6.0, e);
+            },
+            [&](double e) { lt::k_field_update(U, P, e); });
+        const double cv1 = cv_charge(cv, U);
+        const double H1 = hamiltonian(U, P, beta) + bias->potential(cv1);
+        const double d = H1 - H0;
+        int acc = 1;
+        if (metropolis) {
+            const double q = host_uniform(U->seed, counter);
+            acc = q <= std::exp(-d);
+        }
+        if (acc) {
+            bias->cv_current = cv1;
+        } else {
+            bias->cv_current = cv_old;
+            std::fill(samples.begin(), samples.end(), cv_old);
+            copy_field(U, U_copy);
+        }
+        if (update_bias)
+            for (double s : samples) {
+                bias->deposit(s);
+                bias->deposit(-s);
+            }
+        if (dH) *dH = d;
+        if (accepted) *accepted = acc;
+        if (cv_new) *cv_new = acc ? cv1 : cv_old;
+        return LT_OK;
+    });
+}
+
+}  // extern "C"

@@ -11,33 +11,6 @@ namespace {
 
 constexpr int BS = 256;
 
-// Sum of the four clover leaves of plane (mu, nu) at site s (leaf orientation of
-// the reference's CalculateClover<1,1>); leaf I (the plaquette) is returned too.
-__device__ __forceinline__ M3 clover_plane(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int par,
-                                           int mu, int nu, double& retr_plaq)
-{
-    const int q = par ^ 1;
-    const int smu = up(g, s, c[mu], mu), snu = up(g, s, c[nu], nu);
-    const int smd = dn(g, s, c[mu], mu), snd = dn(g, s, c[nu], nu);
-    const int smd_nu = up(g, smd, c[nu], nu), smd_nd = dn(g, smd, c[nu], nu), smu_nd = up(g, snd, c[mu], mu);
-    const M3 Umu = ldu(U, g, s, mu, par), Unu = ldu(U, g, s, nu, par);
-    // leaf I: U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-    M3 l = mul_nd(mul_nd(mul(Umu, ldu(U, g, smu, nu, q)), ldu(U, g, snu, mu, q)), Unu);
-    retr_plaq = l.m[0].re + l.m[4].re + l.m[8].re;
-    M3 C = l;
-    // leaf II: U_nu(x) U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
-    const M3 Umu_md = ldu(U, g, smd, mu, q);
-    l = mul(mul_nd(mul_nd(Unu, ldu(U, g, smd_nu, mu, par)), ldu(U, g, smd, nu, q)), Umu_md);
-    acc(C, l);
-    // leaf III: U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu)
-    l = mul(mul(mul_nd(dag(Umu_md), ldu(U, g, smd_nd, nu, par)), ldu(U, g, smd_nd, mu, par)), ldu(U, g, snd, nu, q));
-    acc(C, l);
-    // leaf IV: U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^+
-    l = mul_nd(mul(mul_dn(ldu(U, g, snd, nu, q), ldu(U, g, snd, mu, q)), ldu(U, g, smu_nd, nu, par)), Umu);
-    acc(C, l);
-    return C;
-}
-
 // F = C - C^+ (anti-hermitian)
 __device__ __forceinline__ M3 ah2(const M3& C)
 {

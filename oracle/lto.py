@@ -277,3 +277,81 @@ def omf2(U, P, shape, beta, tau, n_step):
 
 def hamiltonian(U, P, shape, beta):
     return wilson_action(U, shape, beta) + kinetic(P, shape)
+
+
+class Bias:
+    """MetaBiasPotential (LettuceGauge/actions/bias_potential/metadynamics.hpp:29-507)."""
+
+    def __init__(self, cv_min, cv_max, bins, weight, width, wt_param, threshold, well_tempered=True):
+        if not cv_min < cv_max:
+            raise ValueError("CV_min < CV_max required")      # :70-73
+        if bins <= 0:
+            raise ValueError("bin_number > 0 required")       # :74-77
+        self.cv_min, self.cv_max, self.bins = cv_min, cv_max, int(bins)
+        self.bin_width = (cv_max - cv_min) / bins
+        self.inv = 1.0 / self.bin_width
+        self.weight, self.width, self.wt, self.thr = weight, width, wt_param, threshold
+        self.well_tempered = well_tempered
+        self.grid = np.zeros(self.bins + 1)
+
+    def deposit(self, cv):
+        """UpdatePotential (:103-124)"""
+        x = self.cv_min + np.arange(self.bins + 1) * self.bin_width
+        d = cv - x
+        g = -0.5 / (self.width * self.width) * d * d
+        if self.well_tempered:
+            g = g - self.grid / self.wt
+        self.grid += self.weight * np.exp(g)
+
+    def deposit_symmetric(self, cv):
+        """UpdatePotentialSymmetric (:134-147)"""
+        self.deposit(cv)
+        self.deposit(-cv)
+
+    def potential(self, cv):
+        """ReturnPotential (:151-179)"""
+        i = int(np.floor((cv - self.cv_min) * self.inv))
+        if 0 <= i < self.bins:
+            c = (cv - (self.cv_min + i * self.bin_width)) * self.inv
+            return self.grid[i] * (1.0 - c) + c * self.grid[i + 1]
+        if i < 0:
+            return self.grid[0] + self.thr * (cv - self.cv_min) ** 2
+        return self.grid[-1] + self.thr * (cv - self.cv_max) ** 2
+
+    def derivative(self, cv):
+        """ReturnDerivative (:181-187)"""
+        h = 0.5 * self.bin_width
+        return self.inv * (self.potential(cv + h) - self.potential(cv - h))
+
+
+def metad_omf2(U, P, shape, beta, tau, n_step, rho, n_levels, bias):
+    """HMCMetaDKernel with Integrators::HMC::OMF_2 on given momenta (hmc_metadynamics.hpp:374-459,
+    hmc_gauge.hpp:155-190).  Returns (H_old, H_new, cv_old, cv_new, samples); U, P evolve in place."""
+    eps = tau / n_step
+    lam = 0.1931833275037836
+    alpha, b, gamma = lam * eps, 0.5 * eps, (1.0 - 2.0 * lam) * eps
+    samples = []
+
+    def mom(e, sub):
+        q = cv_charge(U, shape, rho, n_levels)
+        if sub:
+            samples.append(q)
+        _, F = cv_force(U, shape, rho, n_levels, bias.derivative(q))
+        gauge_force_update(U, P, shape, e, beta)
+        P[...] += e * F
+
+    cv_old = cv_charge(U, shape, rho, n_levels)
+    H_old = hamiltonian(U, P, shape, beta) + bias.potential(cv_old)
+    mom(alpha, False)
+    field_update(U, P, shape, b)
+    mom(gamma, False)
+    field_update(U, P, shape, b)
+    for _ in range(n_step - 1):
+        mom(2.0 * alpha, True)
+        field_update(U, P, shape, b)
+        mom(gamma, False)
+        field_update(U, P, shape, b)
+    mom(alpha, True)
+    cv_new = cv_charge(U, shape, rho, n_levels)
+    H_new = hamiltonian(U, P, shape, beta) + bias.potential(cv_new)
+    return H_old, H_new, cv_old, cv_new, samples

@@ -148,6 +148,45 @@ struct InjectedOMF2
     }
 };
 
+// Friend integrator for HMCMetaDKernel: same OMF_2 sequence on injected momenta;
+// H includes the bias at the CV of MetaCharge (hmc_metadynamics.hpp:556-583).
+struct InjectedOMF2MetaD
+{
+    static constexpr bool merged_momentum_updates = true;
+    static constexpr bool path_update_compatible  = true;
+    const GaugeField* P0 {nullptr};
+    double H_old {0.0}, H_new {0.0}, CV_old {0.0}, CV_new {0.0};
+    std::vector<double> samples;
+
+    template<typename HMCFunctor>
+    void operator()(HMCFunctor& HMC, const double trajectory_length, const int n_step)
+    {
+        HMC.Momentum = *P0;
+        CV_old = HMC.MetaCharge();
+        H_old = HMC.Hamiltonian() + HMC.Metapotential.ReturnPotential(CV_old);
+        // OMF_2 (hmc_gauge.hpp:155-190), written out: only this struct is the kernel's friend
+        floatT epsilon {static_cast<floatT>(trajectory_length)/n_step};
+        const double alpha {0.1931833275037836 * epsilon};
+        const double b     {0.5 * epsilon};
+        const double gamma {(1.0 - 2.0 * 0.1931833275037836) * epsilon};
+        HMC.UpdateMomenta(alpha);
+        HMC.UpdateFields(b);
+        HMC.UpdateMomenta(gamma);
+        HMC.UpdateFields(b);
+        for (int step_count = 0; step_count < n_step - 1; ++step_count)
+        {
+            HMC.UpdateMomenta(2.0 * alpha, true);
+            HMC.UpdateFields(b);
+            HMC.UpdateMomenta(gamma);
+            HMC.UpdateFields(b);
+        }
+        HMC.UpdateMomenta(alpha, true);
+        samples = HMC.cv_path_samples;
+        CV_new = HMC.MetaCharge();
+        H_new = HMC.Hamiltonian() + HMC.Metapotential.ReturnPotential(CV_new);
+    }
+};
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -285,6 +324,59 @@ int main(int argc, char** argv)
         HMC(nstep, false);
         write_field(argv[4], *U);
         std::printf("H_old %.17g\nH_new %.17g\n", integ.H_old, integ.H_new);
+    } else if (cmd == "metad_hmc") {             // metad_hmc in P out gridout beta tau nstep rho cvmin cvmax bins w width wt thr ndep dep...
+        read_field(argv[2], *U);
+        auto P = std::make_unique<GaugeField>();
+        read_field(argv[3], *P);
+        set_beta(std::atof(argv[6]));
+        double tau = std::atof(argv[7]);
+        int nstep = std::atoi(argv[8]);
+        double rho = std::atof(argv[9]);
+        MetaBiasPotential B(std::atof(argv[10]), std::atof(argv[11]), std::atoi(argv[12]), std::atof(argv[13]),
+                            std::atof(argv[14]), std::atof(argv[15]), std::atof(argv[16]));
+        int k = 17;
+        int ndep = std::atoi(argv[k++]);
+        std::vector<double> deps;
+        for (int i = 0; i < ndep; ++i) deps.push_back(std::atof(argv[k++]));
+        B.UpdatePotentialSymmetric(deps);
+        auto Ucopy = std::make_unique<GaugeField>();
+        auto Mom = std::make_unique<GaugeField>();
+        pcg64 g(1);
+        PRNG4D<Nt, Nx, Ny, Nz, pcg64, floatT, int> prng(g);
+        RunStatistics stats;
+        std::ofstream devnull("/dev/null");
+        std::string logpath = "/dev/null";
+        GaugeUpdates::HMCMetaDData data(n_smear_meta);
+        InjectedOMF2MetaD integ;
+        integ.P0 = P.get();
+        auto act = GaugeAction::WilsonAction;
+        GaugeUpdates::HMCMetaDKernel HMC(*U, *Ucopy, *Mom, B, data, integ, act, prng, stats, devnull, logpath, tau, rho);
+        HMC(nstep, false);
+        write_field(argv[4], *U);
+        std::vector<double> grid;
+        for (int i = 0; i <= std::atoi(argv[12]); ++i)
+            grid.push_back(B.ReturnPotential(std::atof(argv[10]) + i * (std::atof(argv[11]) - std::atof(argv[10])) / std::atoi(argv[12])));
+        write_values(argv[5], grid);
+        std::printf("H_old %.17g\nH_new %.17g\nCV_old %.17g\nCV_new %.17g\n", integ.H_old, integ.H_new, integ.CV_old, integ.CV_new);
+        for (double s : integ.samples) std::printf("sample %.17g\n", s);
+    } else if (cmd == "bias") {                  // bias out cvmin cvmax bins weight width wt thr ndep dep... neval eval...
+        // MetaBiasPotential (metadynamics.hpp:29-507); the reference compiles the
+        // well-tempered rule in (defines.hpp:72)
+        MetaBiasPotential B(std::atof(argv[3]), std::atof(argv[4]), std::atoi(argv[5]), std::atof(argv[6]),
+                            std::atof(argv[7]), std::atof(argv[8]), std::atof(argv[9]));
+        int k = 10;
+        int ndep = std::atoi(argv[k++]);
+        std::vector<double> deps;
+        for (int i = 0; i < ndep; ++i) deps.push_back(std::atof(argv[k++]));
+        B.UpdatePotentialSymmetric(deps);
+        int neval = std::atoi(argv[k++]);
+        std::vector<double> out;
+        for (int i = 0; i < neval; ++i) {
+            double cv = std::atof(argv[k++]);
+            out.push_back(B.ReturnPotential(cv));
+            out.push_back(B.ReturnDerivative(cv));
+        }
+        write_values(argv[2], out);
     } else if (cmd == "hb_stats") {              // hb_stats out beta n_update n_or seed [in]
         set_beta(std::atof(argv[3]));
         int nupd = std::atoi(argv[4]);

@@ -1,0 +1,256 @@
+"""HMC, the collective variable with its hand-written adjoint, and MetaD-HMC on the
+GPU (through the C ABI) against the CPU oracle and the reference's golden vectors."""
+import numpy as np
+import pytest
+
+from conftest import assert_field_matches, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SMALL = (8, 8, 8, 8)
+TINY = (4, 4, 4, 4)
+G16 = (16, 16, 16, 16)
+
+
+def flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def la():
+    import lettuce_amd
+    return lettuce_amd
+
+
+def smooth(lto, shape, seed=0, n=3):
+    return lto.stout(lto.haar(shape, seed), shape, 0.1, n)
+
+
+def gauge(la, ctx, shape, data=None):
+    f = la.GaugeField(ctx, shape)
+    if data is not None:
+        f.upload(data)
+    return f
+
+
+def matrix(la, ctx, shape, data=None):
+    f = la.MatrixField(ctx, shape)
+    if data is not None:
+        f.upload(data)
+    return f
+
+
+def test_momenta_match_oracle_stream(la, lto, gpu_ctx):
+    flush("momenta")
+    P = matrix(la, gpu_ctx, SMALL)
+    la.RandomMomentum(P, 17, 3)
+    ref = lto.random_momentum(SMALL, 17, 3)
+    assert rel_err(P.download(), ref) <= 1e-13
+
+
+def test_force_and_field_updates(la, lto, gpu_ctx):
+    flush("force/field update")
+    U0 = smooth(lto, SMALL, 1)
+    P0 = lto.random_momentum(SMALL, 2, 0)
+    U = gauge(la, gpu_ctx, SMALL, U0)
+    P = matrix(la, gpu_ctx, SMALL, P0)
+    la.UpdateMomenta(U, P, 0.1, 6.0)
+    la.UpdateFields(U, P, 0.05)
+    Ur, Pr = U0.copy(), P0.copy()
+    lto.gauge_force_update(Ur, Pr, SMALL, 0.1, 6.0)
+    lto.field_update(Ur, Pr, SMALL, 0.05)
+    assert rel_err(P.download(), Pr) <= 1e-13
+    assert rel_err(U.download(), Ur) <= 1e-13
+    H = la.Hamiltonian(U, P, 6.0)
+    assert abs(H - lto.hamiltonian(Ur, Pr, SMALL, 6.0)) <= 1e-12 * abs(H)
+
+
+@pytest.mark.parametrize("integrator", ["leapfrog", "omf2", "omf4"])
+def test_integrators_match_oracle(la, lto, gpu_ctx, integrator):
+    flush(f"integrate {integrator}")
+    U0 = smooth(lto, SMALL, 3)
+    P0 = lto.random_momentum(SMALL, 4, 0)
+    U = gauge(la, gpu_ctx, SMALL, U0)
+    P = matrix(la, gpu_ctx, SMALL, P0)
+    la.Integrate(U, P, 6.0, integrator, 0.5, 6)
+    if integrator == "omf2":
+        Ur, Pr = U0.copy(), P0.copy()
+        lto.omf2(Ur, Pr, SMALL, 6.0, 0.5, 6)
+        assert rel_err(U.download(), Ur) <= 1e-10
+        assert rel_err(P.download(), Pr) <= 1e-10
+    # reversibility (hmc_gauge.hpp:585-599): flip momenta, integrate back
+    Pm = -P.download()
+    P.upload(Pm)
+    la.Integrate(U, P, 6.0, integrator, 0.5, 6)
+    assert np.max(np.abs(U.download() - U0)) <= 1e-10
+    assert np.max(np.abs(-P.download() - P0)) <= 1e-10
+
+
+def test_omf2_trajectory_golden16(la, lto, golden, gpu_ctx):
+    flush("omf2 golden 16^4")
+    U0 = lto.stout(lto.haar(G16, 0), G16, 0.1, 3)
+    U = gauge(la, gpu_ctx, G16, U0)
+    P = matrix(la, gpu_ctx, G16, lto.random_momentum(G16, 5, 0))
+    H0 = la.Hamiltonian(U, P, 6.0)
+    la.Integrate(U, P, 6.0, "omf2", 1.0, 10)
+    H1 = la.Hamiltonian(U, P, 6.0)
+    Href = golden["hmc_omf2_smooth_H"]
+    assert abs(H0 - Href[0]) <= 1e-12 * abs(Href[0])
+    assert abs((H1 - H0) - (Href[1] - Href[0])) <= 1e-7 * max(1.0, abs(Href[1] - Href[0]))
+    assert_field_matches(golden, "hmc_omf2_smooth", U.download(), 1e-9)
+
+
+def test_trajectory_metropolis_and_momentum_stream(la, lto, gpu_ctx):
+    flush("hmc trajectory")
+    U0 = smooth(lto, SMALL, 5)
+    U = gauge(la, gpu_ctx, SMALL, U0)
+    Uc = gauge(la, gpu_ctx, SMALL)
+    P = matrix(la, gpu_ctx, SMALL)
+    U.set_rng(9, 7)
+    hmc = la.HMCKernel(U, Uc, P, "omf2", 6.0, 1.0)
+    acc = hmc(8, metropolis_step=False)
+    assert acc and U.get_rng() == (9, 8)
+    # same trajectory on the oracle from the same Philox momenta
+    Ur = U0.copy()
+    Pr = lto.random_momentum(SMALL, 9, 7)
+    H0 = lto.hamiltonian(Ur, Pr, SMALL, 6.0)
+    lto.omf2(Ur, Pr, SMALL, 6.0, 1.0, 8)
+    dH = lto.hamiltonian(Ur, Pr, SMALL, 6.0) - H0
+    assert abs(hmc.delta_H - dH) <= 1e-8 * max(1.0, abs(dH))
+    assert rel_err(U.download(), Ur) <= 1e-10
+    # rejection restores the field exactly
+    before = U.download()
+    hmc.trajectory_length = 8.0     # far too long a trajectory: certain rejection
+    acc = hmc(4, metropolis_step=True)
+    if not acc:
+        assert np.array_equal(U.download(), before)
+
+
+@pytest.mark.parametrize("n_levels", [0, 2, 4])
+def test_cv_forward_backward_vs_oracle(la, lto, gpu_ctx, n_levels):
+    flush(f"cv levels {n_levels}")
+    U0 = smooth(lto, SMALL, 6)
+    U = gauge(la, gpu_ctx, SMALL, U0)
+    F = matrix(la, gpu_ctx, SMALL)
+    cv = la.CollectiveVariable(gpu_ctx, SMALL, n_levels, 0.12)
+    Q = cv.forward(U)
+    cv.backward(0.7, F)
+    Qr, Fr = lto.cv_force(U0, SMALL, 0.12, n_levels, 0.7)
+    assert abs(Q - Qr) <= 1e-12 * max(1.0, abs(Qr))
+    assert rel_err(F.download(), Fr) <= 1e-11
+    cv.close()
+
+
+def test_cv_force_golden16(la, lto, golden, gpu_ctx):
+    flush("cv golden 16^4")
+    U0 = lto.stout(lto.haar(G16, 0), G16, 0.1, 3)
+    U = gauge(la, gpu_ctx, G16, U0)
+    F = matrix(la, gpu_ctx, G16)
+    cv = la.CollectiveVariable(gpu_ctx, G16, 4, 0.12)
+    Q = cv.forward(U)
+    cv.backward(1.0, F)
+    assert abs(Q - golden["cv_force_smooth_Q"][0]) <= 1e-12 * max(1.0, abs(Q))
+    assert_field_matches(golden, "cv_force_smooth", F.download(), 1e-11)
+    cv.close()
+
+
+def test_cv_force_finite_differences_on_gpu(la, lto, gpu_ctx):
+    """Central differences of the GPU charge against the GPU adjoint on random links:
+    dQ/dh of U_l -> exp(i h/2 lambda_a) U_l = Re tr(lambda_a i F_l).  h = 1e-5 balances truncation
+    (~h^2) and rounding (~1e-13/h) errors; h = 1e-6 (the north-star setting) is rounding-limited."""
+    flush("cv FD")
+    from test_oracle_golden import gell_mann
+    U0 = smooth(lto, SMALL, 7, 4)
+    U = gauge(la, gpu_ctx, SMALL, U0)
+    F = matrix(la, gpu_ctx, SMALL)
+    cv = la.CollectiveVariable(gpu_ctx, SMALL, 4, 0.12)
+    cv.forward(U)
+    cv.backward(1.0, F)
+    Fm = F.download().reshape(-1, 3, 3, 2)
+    Fm = Fm[..., 0] + 1j * Fm[..., 1]
+    lam = gell_mann()
+    flat = U0.reshape(-1, 18)
+    rng = np.random.default_rng(1)
+    links = rng.choice(flat.shape[0], 8, replace=False)
+    for h, tol in ((1e-5, 2e-6), (1e-6, 2e-5)):
+        worst = fd_worst(la, lto, U, U0, cv, Fm, lam, flat, links, h)
+        print(f"h={h:g} worst relative FD deviation {worst:.2e}", flush=True)
+        assert worst <= tol, (h, worst)
+    cv.close()
+
+
+def fd_worst(la, lto, U, U0, cv, Fm, lam, flat, links, h):
+    worst = 0.0
+    for link in links:
+        M = flat[link].reshape(3, 3, 2)
+        M = M[..., 0] + 1j * M[..., 1]
+        for a in range(8):
+            vals = []
+            for sgn in (1, -1):
+                E = lto.su3_exp(np.stack([(sgn * 0.5 * h * lam[a]).real, (sgn * 0.5 * h * lam[a]).imag], -1))
+                E = E.reshape(3, 3, 2)
+                W = U0.copy()
+                N = (E[..., 0] + 1j * E[..., 1]) @ M
+                W.reshape(-1, 18)[link] = np.stack([N.real, N.imag], -1).reshape(18)
+                U.upload(W)
+                vals.append(cv.forward(U))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            an = np.real(np.trace(lam[a] @ (1j * Fm[link])))
+            scale = max(abs(an), 1e-4)
+            worst = max(worst, abs(fd - an) / scale)
+    return worst
+
+
+def metad_setup(la, lto, golden):
+    a = golden["metad_bias_args"]
+    bias = la.MetaBiasPotential(a[0], a[1], int(a[2]), a[3], a[4], a[5], a[6], well_tempered=True)
+    obias = lto.Bias(a[0], a[1], int(a[2]), a[3], a[4], a[5], a[6], well_tempered=True)
+    for d in golden["metad_bias_deposits"]:
+        bias.UpdatePotentialSymmetric(d)
+        obias.deposit_symmetric(d)
+    return bias, obias
+
+
+def test_metad_trajectory_golden16(la, lto, golden, gpu_ctx):
+    """MetaD-HMC OMF2 trajectory with Philox momenta equal to the injected ones of the golden run."""
+    flush("metad golden 16^4")
+    bias, _ = metad_setup(la, lto, golden)
+    U0 = lto.stout(lto.haar(G16, 0), G16, 0.1, 3)
+    U = gauge(la, gpu_ctx, G16, U0)
+    Uc = gauge(la, gpu_ctx, G16)
+    P = matrix(la, gpu_ctx, G16)
+    F = matrix(la, gpu_ctx, G16)
+    cv = la.CollectiveVariable(gpu_ctx, G16, 4, 0.12)
+    U.set_rng(5, 0)     # momenta = lto_random_momentum(seed=5, counter=0), as injected in the reference
+    k = la.HMCMetaDKernel(U, Uc, P, F, bias, cv, "omf2", 6.0, 0.5, update_bias=True)
+    assert k(5, metropolis_step=False)
+    Href = golden["metad_omf2_smooth_H"]
+    assert abs(k.delta_H - (Href[1] - Href[0])) <= 1e-7 * max(1.0, abs(Href[1] - Href[0]))
+    assert abs(k.cv_value - golden["metad_omf2_smooth_CV"][1]) <= 1e-9
+    assert_field_matches(golden, "metad_omf2_smooth", U.download(), 1e-9)
+    a = golden["metad_bias_args"]
+    x = a[0] + np.arange(int(a[2]) + 1) * (a[1] - a[0]) / int(a[2])
+    got = np.array([bias.ReturnPotential(v) for v in x])
+    assert rel_err(got, golden["metad_omf2_smooth_grid_after"]) <= 1e-9
+    cv.close()
+
+
+def test_metad_trajectory_vs_oracle_small(la, lto, golden, gpu_ctx):
+    flush("metad 4^4 vs oracle")
+    bias, obias = metad_setup(la, lto, golden)
+    U0 = smooth(lto, TINY, 8, 2)
+    U = gauge(la, gpu_ctx, TINY, U0)
+    Uc = gauge(la, gpu_ctx, TINY)
+    P = matrix(la, gpu_ctx, TINY)
+    F = matrix(la, gpu_ctx, TINY)
+    cv = la.CollectiveVariable(gpu_ctx, TINY, 2, 0.12)
+    U.set_rng(3, 11)
+    k = la.HMCMetaDKernel(U, Uc, P, F, bias, cv, "omf2", 6.0, 0.4, update_bias=False)
+    k(4, metropolis_step=False)
+    Ur = U0.copy()
+    Pr = lto.random_momentum(TINY, 3, 11)
+    H0, H1, _, cv1, _ = lto.metad_omf2(Ur, Pr, TINY, 6.0, 0.4, 4, 0.12, 2, obias)
+    assert abs(k.delta_H - (H1 - H0)) <= 1e-8 * max(1.0, abs(H1 - H0))
+    assert abs(k.cv_value - cv1) <= 1e-10
+    assert rel_err(U.download(), Ur) <= 1e-10
+    cv.close()

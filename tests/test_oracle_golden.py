@@ -119,6 +119,31 @@ def test_hmc_injected_momenta(lto, golden, configs):
     assert_field_matches(golden, "hmc_omf2_smooth", U, 1e-9)
 
 
+def test_metad_trajectory_injected_momenta(lto, golden, configs):
+    """HMCMetaDKernel OMF2 trajectory (tau = 0.5, 5 steps, 4 stout levels, rho = 0.12) on injected
+    momenta with a deposited well-tempered bias, against the reference."""
+    a = golden["metad_bias_args"]
+    bias = lto.Bias(a[0], a[1], int(a[2]), a[3], a[4], a[5], a[6], well_tempered=True)
+    for d in golden["metad_bias_deposits"]:
+        bias.deposit_symmetric(d)
+    U = configs["smooth"].copy()
+    P = lto.random_momentum(SHAPE, 5, 0)
+    H0, H1, cv0, cv1, samples = lto.metad_omf2(U, P, SHAPE, 6.0, 0.5, 5, 0.12, 4, bias)
+    Href = golden["metad_omf2_smooth_H"]
+    cvref = golden["metad_omf2_smooth_CV"]
+    assert abs(H0 - Href[0]) <= 1e-12 * abs(Href[0])
+    assert abs((H1 - H0) - (Href[1] - Href[0])) <= 1e-7 * max(1.0, abs(Href[1] - Href[0]))
+    assert abs(cv0 - cvref[0]) <= 1e-11 and abs(cv1 - cvref[1]) <= 1e-9
+    assert np.max(np.abs(np.array(samples) - golden["metad_omf2_smooth_samples"])) <= 1e-9
+    assert_field_matches(golden, "metad_omf2_smooth", U, 1e-9)
+    # path update: the reference deposits every sample symmetrically after the trajectory
+    for s in samples:
+        bias.deposit_symmetric(s)
+    x = a[0] + np.arange(int(a[2]) + 1) * (a[1] - a[0]) / int(a[2])
+    got = np.array([bias.potential(v) for v in x])
+    assert rel_err(got, golden["metad_omf2_smooth_grid_after"]) <= 1e-9
+
+
 def gell_mann():
     l = np.zeros((8, 3, 3), dtype=complex)
     l[0][0, 1] = l[0][1, 0] = 1
