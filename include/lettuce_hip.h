@@ -189,6 +189,55 @@ int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F,
                             double beta, int integrator, double tau, int n_step, int metropolis,
                             int update_bias, double *dH, int *accepted, double *cv_new);
 
+/* ---- replica batches and parallel-tempering metadynamics ------------------ */
+/* The reference tempers two streams, an unbiased one and one MetaD-HMC stream
+ * with the full bias, swapping them with probability min(1, exp(-(V(Q_unbiased)
+ * - V(Q_biased)))) (MetadynamicsTemperingKernel, updates/tempering.hpp:24-73,
+ * driven from SU3_main.cpp:549-621).  A batch generalises this to n replicas of
+ * equal or mixed extents run by the same launches: replica r runs MetaD-HMC with
+ * bias strength[label[r]] * V(Q); adjacent labels k, k+1 held by replicas of
+ * equal extents swap with probability min(1, exp(-(s_k - s_{k+1}) (V(Q_j) - V(Q_i)))),
+ * which for s = (0, 1) is the reference rule.  The replica holding the top label
+ * deposits its path samples (as HMCMetaDKernel does, hmc_metadynamics.hpp:615-650).
+ * Defaults: replica r has seed r, counter 0, label r, strength r / (n - 1). */
+typedef struct lt_batch_s *lt_batch;
+
+/* swap_mode of lt_batch_metad_hmc */
+#define LT_SWAP_NONE 0
+#define LT_SWAP_EVEN 1   /* label pairs (0,1), (2,3), ... */
+#define LT_SWAP_ODD  2   /* label pairs (1,2), (3,4), ... */
+#define LT_SWAP_ALT  3   /* even and odd pairs on alternating steps */
+
+/* shapes: n_rep * {Nt, Nx, Ny, Nz}; n_levels stout levels of parameter rho for the CV
+ * (the reference's n_smear_meta and rho_stout_metadynamics); fields start at identity */
+int lt_batch_create(lt_ctx ctx, int n_rep, const int *shapes, int n_levels, double rho, lt_batch *out);
+int lt_batch_destroy(lt_batch b);
+int lt_batch_size(lt_batch b, int *n_rep);
+/* the gauge field of replica r as a field handle owned by the batch: every
+ * single-lattice entry point accepts it (not lt_field_destroy / lt_field_swap) */
+int lt_batch_field(lt_batch b, int r, lt_field *view);
+/* Philox key and trajectory counter of replica r used by batched trajectories */
+int lt_batch_set_rng(lt_batch b, int r, uint64_t seed, uint64_t counter);
+int lt_batch_get_rng(lt_batch b, int r, uint64_t *seed, uint64_t *counter);
+/* strengths[k] of label k (n_rep values), labels (NULL: identity), exchange RNG state */
+int lt_batch_set_ladder(lt_batch b, const double *strengths, const int *labels, uint64_t pt_seed,
+                        uint64_t pt_counter);
+/* copy a host bias (parameters, grid, current CV) into the batch / the batch's grid back */
+int lt_batch_set_bias(lt_batch b, lt_bias bias);
+int lt_batch_get_bias(lt_batch b, lt_bias bias);
+/* enable (1) or disable (0) replaying each step as one captured hipGraph */
+int lt_batch_use_graph(lt_batch b, int enable);
+/* n_traj PT-MetaD steps: one trajectory per replica each, then deposits (update_bias)
+ * and exchanges (swap_mode); enqueues only */
+int lt_batch_metad_hmc(lt_batch b, double beta, int integrator, double tau, int n_step, int metropolis,
+                       int update_bias, int swap_mode, int n_traj);
+/* last step, per replica (NULL entries skipped): dH, accepted, CV after the step, label */
+int lt_batch_results(lt_batch b, double *dH, int *accepted, double *cv, int *labels);
+/* last step's CV path samples, n_rep rows of (samples per trajectory) */
+int lt_batch_samples(lt_batch b, double *samples, int n);
+/* cumulative exchange statistics: stats[2k] proposed, stats[2k+1] accepted for labels (k, k+1) */
+int lt_batch_swap_stats(lt_batch b, long long *stats, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -481,3 +481,117 @@ class HMCMetaDKernel:
         self.delta_H = d.value
         self.cv_value = q.value
         return bool(a.value)
+
+
+class _ReplicaField(GaugeField):
+    """Gauge field of one replica of a ReplicaBatch; the batch owns its memory."""
+
+    def __init__(self, ctx, shape, handle, batch):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self._h = handle
+        self._batch = batch   # keeps the batch alive while the view is used
+
+    def close(self):
+        self._h = None
+
+
+SWAP_MODES = {"none": 0, "even": 1, "odd": 2, "alternate": 3}
+
+
+class ReplicaBatch:
+    """n lattices (equal or mixed extents) advanced by the same launches (lt_batch).
+
+    Generalises the reference's two-stream tempering (MetadynamicsTemperingKernel,
+    updates/tempering.hpp:24-73, SU3_main.cpp:549-621) to a ladder of bias
+    strengths: replica r runs MetaD-HMC (HMCMetaDKernel) with bias
+    strengths[label[r]] * V(Q); adjacent labels exchange with probability
+    min(1, exp(-(s_k - s_k+1) (V(Q_j) - V(Q_i)))); the replica with the top label
+    deposits its path samples into the shared bias.
+    """
+
+    def __init__(self, ctx, shapes, n_levels=4, rho=0.12):
+        self.ctx = ctx
+        self.shapes = [tuple(int(x) for x in s) for s in shapes]
+        flat = (ctypes.c_int * (4 * len(self.shapes)))(*[x for s in self.shapes for x in s])
+        h = ctypes.c_void_p()
+        call("lt_batch_create", ctx.handle, len(self.shapes), flat, n_levels, rho, ctypes.byref(h))
+        self._h = h
+        self.n = len(self.shapes)
+        self._n_sub = 0
+        self._views = []
+        for r, s in enumerate(self.shapes):
+            f = ctypes.c_void_p()
+            call("lt_batch_field", h, r, ctypes.byref(f))
+            self._views.append(_ReplicaField(ctx, s, f, self))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def field(self, r):
+        return self._views[r]
+
+    def set_rng(self, r, seed, counter=0):
+        call("lt_batch_set_rng", self._h, r, seed, counter)
+
+    def get_rng(self, r):
+        s = ctypes.c_uint64()
+        c = ctypes.c_uint64()
+        call("lt_batch_get_rng", self._h, r, ctypes.byref(s), ctypes.byref(c))
+        return s.value, c.value
+
+    def set_ladder(self, strengths, labels=None, pt_seed=0, pt_counter=0):
+        s = np.ascontiguousarray(strengths, dtype=np.float64)
+        if s.size != self.n:
+            raise ValueError("one strength per replica")
+        lab = None
+        if labels is not None:
+            lab = (ctypes.c_int * self.n)(*[int(x) for x in labels])
+        call("lt_batch_set_ladder", self._h, _dp(s), lab, pt_seed, pt_counter)
+
+    def set_bias(self, bias):
+        call("lt_batch_set_bias", self._h, bias.handle)
+
+    def get_bias(self, bias):
+        call("lt_batch_get_bias", self._h, bias.handle)
+
+    def use_graph(self, enable=True):
+        call("lt_batch_use_graph", self._h, 1 if enable else 0)
+
+    def metad_hmc(self, n_step, beta=6.0, integrator="omf2", tau=1.0, metropolis=True, update_bias=False,
+                  swap="none", n_traj=1):
+        call("lt_batch_metad_hmc", self._h, beta, _INTEGRATORS[integrator], tau, n_step, 1 if metropolis else 0,
+             1 if update_bias else 0, SWAP_MODES[swap], n_traj)
+        self._n_sub = n_step
+
+    def results(self):
+        dH = np.zeros(self.n)
+        cv = np.zeros(self.n)
+        acc = (ctypes.c_int * self.n)()
+        lab = (ctypes.c_int * self.n)()
+        call("lt_batch_results", self._h, _dp(dH), acc, _dp(cv), lab)
+        return {"dH": dH, "accepted": np.array(acc[:], dtype=bool), "cv": cv, "labels": np.array(lab[:])}
+
+    def samples(self):
+        a = np.zeros((self.n, self._n_sub))
+        call("lt_batch_samples", self._h, _dp(a), a.size)
+        return a
+
+    def swap_stats(self):
+        a = (ctypes.c_longlong * (2 * self.n))()
+        call("lt_batch_swap_stats", self._h, a, 2 * self.n)
+        return np.array(a[:], dtype=np.int64).reshape(self.n, 2)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            for v in self._views:
+                v._h = None
+            call("lt_batch_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -74,6 +74,20 @@ SIGNATURES = {
     "lt_bias_cv": [ctypes.c_void_p, D, I],
     "lt_metad_hmc_trajectory": [c_field, c_field, c_field, c_field, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                 I, ctypes.c_double, I, I, I, D, ctypes.POINTER(I), D],
+    "lt_batch_create": [c_ctx, I, ctypes.POINTER(I), I, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)],
+    "lt_batch_destroy": [ctypes.c_void_p],
+    "lt_batch_size": [ctypes.c_void_p, ctypes.POINTER(I)],
+    "lt_batch_field": [ctypes.c_void_p, I, ctypes.POINTER(c_field)],
+    "lt_batch_set_rng": [ctypes.c_void_p, I, ctypes.c_uint64, ctypes.c_uint64],
+    "lt_batch_get_rng": [ctypes.c_void_p, I, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
+    "lt_batch_set_ladder": [ctypes.c_void_p, D, ctypes.POINTER(I), ctypes.c_uint64, ctypes.c_uint64],
+    "lt_batch_set_bias": [ctypes.c_void_p, ctypes.c_void_p],
+    "lt_batch_get_bias": [ctypes.c_void_p, ctypes.c_void_p],
+    "lt_batch_use_graph": [ctypes.c_void_p, I],
+    "lt_batch_metad_hmc": [ctypes.c_void_p, ctypes.c_double, I, ctypes.c_double, I, I, I, I, I],
+    "lt_batch_results": [ctypes.c_void_p, D, ctypes.POINTER(I), D, ctypes.POINTER(I)],
+    "lt_batch_samples": [ctypes.c_void_p, D, I],
+    "lt_batch_swap_stats": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), I],
 }
 
 

@@ -146,25 +146,8 @@ void lt_ctx_s::collect_profile()
 
 namespace {
 
-template <typename F>
-int guarded(F&& f)
-{
-    try {
-        return f();
-    } catch (const lt::LtError& e) {
-        lt::set_error(e.msg);
-        return e.code;
-    } catch (const std::exception& e) {
-        lt::set_error(e.what());
-        return LT_ESTATE;
-    }
-}
-
-int fail(int code, const std::string& msg)
-{
-    lt::set_error(msg);
-    return code;
-}
+using lt::fail;
+using lt::guarded;
 
 void use_device(lt_ctx ctx) { LT_HIP_CHECK(hipSetDevice(ctx->device)); }
 
@@ -300,6 +283,7 @@ int lt_field_create(lt_ctx ctx, int nt, int nx, int ny, int nz, lt_field* out)
 int lt_field_destroy(lt_field f)
 {
     if (!f) return LT_OK;
+    if (f->view) return fail(LT_ESTATE, "lt_field_destroy: this field is a replica of a batch (destroy the batch)");
     return guarded([&] {
         use_device(f->ctx);
         LT_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
@@ -382,6 +366,7 @@ int lt_field_swap(lt_field a, lt_field b)
 {
     if (!a || !b) return fail(LT_EINVAL, "lt_field_swap: null argument");
     if (!same_shape(a, b) || a->ctx != b->ctx) return fail(LT_ESTATE, "lt_field_swap: shape/context mismatch");
+    if (a->view || b->view) return fail(LT_ESTATE, "lt_field_swap: replicas of a batch cannot be swapped");
     std::swap(a->d, b->d);
     std::swap(a->su3, b->su3);
     return LT_OK;

@@ -500,13 +500,16 @@ __device__ __forceinline__ bool map_site(const Geom& g, const LinkMap& m, int pa
 // Jacobi kernels over all links: the 8 (mu, parity) classes of one tile are
 // consecutive blocks of the same XCD, so the staples of all directions at a tile
 // are built from one L2-resident neighbourhood.  Grid: 8 * tiles blocks.
-__device__ __forceinline__ bool map_link_all(const Geom& g, const LinkMap& m, int& mu, int& par, int c[4], int& s)
+__device__ __forceinline__ bool map_link_b(const Geom& g, const LinkMap& m, int b, int& mu, int& par, int c[4], int& s)
 {
-    int b = xcd_block(blockIdx.x, gridDim.x);
     int mp = b & 7;
     mu = mp >> 1;
     par = mp & 1;
     return map_site_b(g, m, b >> 3, par, c, s);
+}
+__device__ __forceinline__ bool map_link_all(const Geom& g, const LinkMap& m, int& mu, int& par, int c[4], int& s)
+{
+    return map_link_b(g, m, xcd_block(blockIdx.x, gridDim.x), mu, par, c, s);
 }
 
 // Site of the cb-th entry of a parity: s = 2*cb or 2*cb+1 with the right parity.

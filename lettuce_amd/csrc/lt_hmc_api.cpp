@@ -42,74 +42,11 @@ struct lt_cv_s {
     bool forward_done = false;
 };
 
-// MetaBiasPotential (LettuceGauge/actions/bias_potential/metadynamics.hpp:29-507)
-struct lt_bias_s {
-    std::vector<double> grid;
-    double cv_current = 0.0, cv_min = 0.0, cv_max = 0.0, bin_width = 0.0, bin_width_inverse = 0.0;
-    double weight = 0.0, width = 0.0, wt_param = 0.0, threshold = 0.0;
-    int bins = 0;
-    int well_tempered = 0;
-    unsigned long long exceeded = 0;
-
-    int bin_of(double cv) const { return (int)std::floor((cv - cv_min) * bin_width_inverse); }
-    double cv_of(int i) const { return cv_min + i * bin_width; }
-    // UpdatePotential (metadynamics.hpp:103-124)
-    void deposit(double cv)
-    {
-        if (cv < cv_min || cv >= cv_max) exceeded += 1;
-        const double wis = 1.0 / (width * width);
-        for (size_t b = 0; b < grid.size(); ++b) {
-            const double dist = cv - cv_of((int)b);
-            if (well_tempered)
-                grid[b] += weight * std::exp(-0.5 * wis * dist * dist - grid[b] / wt_param);
-            else
-                grid[b] += weight * std::exp(-0.5 * wis * dist * dist);
-        }
-    }
-    // ReturnPotential (metadynamics.hpp:151-179)
-    double potential(double cv) const
-    {
-        const int i = bin_of(cv);
-        if ((unsigned)i < (unsigned)(bins)) {
-            const double ic = (cv - cv_of(i)) * bin_width_inverse;
-            return grid[i] * (1.0 - ic) + ic * grid[i + 1];
-        }
-        if (i < 0) {
-            const double d = cv - cv_min;
-            return grid.front() + threshold * d * d;
-        }
-        const double d = cv - cv_max;
-        return grid.back() + threshold * d * d;
-    }
-    // ReturnDerivative (metadynamics.hpp:181-187)
-    double derivative(double cv) const
-    {
-        const double h = 0.5 * bin_width;
-        return bin_width_inverse * (potential(cv + h) - potential(cv - h));
-    }
-};
-
 namespace {
 
-template <typename F>
-int guarded(F&& f)
-{
-    try {
-        return f();
-    } catch (const lt::LtError& e) {
-        lt::set_error(e.msg);
-        return e.code;
-    } catch (const std::exception& e) {
-        lt::set_error(e.what());
-        return LT_ESTATE;
-    }
-}
-
-int fail(int code, const std::string& msg)
-{
-    lt::set_error(msg);
-    return code;
-}
+using lt::fail;
+using lt::guarded;
+using lt::valid_integrator;
 
 bool same_shape(const lt::Geom& a, const lt::Geom& b)
 {
@@ -148,11 +85,13 @@ double host_uniform(uint64_t seed, uint64_t counter)
     return ((double)(c[0] >> 5) * 67108864.0 + (double)(c[1] >> 6)) * (1.0 / 9007199254740992.0);
 }
 
+}  // namespace
+
 // Integrators::HMC::{Leapfrog, OMF_2, OMF_4} (LettuceGauge/updates/hmc_gauge.hpp:48-75,
 // 155-190, 241-295); momentum updates carry the "submeasurement" flag the MetaD
 // path update uses.
-void integrate(int kind, double tau, int n_step, const std::function<void(double, bool)>& P,
-               const std::function<void(double)>& U)
+void lt::hmc_schedule(int kind, double tau, int n_step, const std::function<void(double, bool)>& P,
+                      const std::function<void(double)>& U)
 {
     if (n_step <= 0) return;
     const double eps = tau / n_step;
@@ -204,7 +143,13 @@ void integrate(int kind, double tau, int n_step, const std::function<void(double
     }
 }
 
-bool valid_integrator(int k) { return k == LT_INT_LEAPFROG || k == LT_INT_OMF2 || k == LT_INT_OMF4; }
+namespace {
+
+void integrate(int kind, double tau, int n_step, const std::function<void(double, bool)>& P,
+               const std::function<void(double)>& U)
+{
+    lt::hmc_schedule(kind, tau, n_step, P, U);
+}
 
 double hamiltonian(lt_field U, lt_field P, double beta)
 {

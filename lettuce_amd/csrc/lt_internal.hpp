@@ -5,7 +5,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
+#include <exception>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -28,6 +31,27 @@ struct LtError {
         if (e_ != hipSuccess)                                                                       \
             throw ::lt::LtError{LT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)};        \
     } while (0)
+
+// C ABI bodies: exceptions become a status code plus the lt_last_error message.
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const LtError& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return LT_ESTATE;
+    }
+}
+
+inline int fail(int code, const std::string& msg)
+{
+    set_error(msg);
+    return code;
+}
 
 struct ProfEntry {
     int launches = 0;
@@ -60,10 +84,58 @@ struct lt_field_s {
     lt::Geom g{};
     lt::LinkMap map{};         // thread -> site tiling of the per-parity link kernels
     bool su3 = true;           // rows 0,1 stored, row 2 implied (false: general 3x3 matrices)
+    bool view = false;         // d belongs to a replica batch (not freed, not swappable)
     double2* d = nullptr;      // 4 * 2 * 9 * Vh complex doubles
     uint64_t seed = 0;
     uint64_t counter = 0;
     size_t bytes() const { return (size_t)72 * g.Vh * sizeof(double2); }
+};
+
+// MetaBiasPotential (LettuceGauge/actions/bias_potential/metadynamics.hpp:29-507)
+struct lt_bias_s {
+    std::vector<double> grid;
+    double cv_current = 0.0, cv_min = 0.0, cv_max = 0.0, bin_width = 0.0, bin_width_inverse = 0.0;
+    double weight = 0.0, width = 0.0, wt_param = 0.0, threshold = 0.0;
+    int bins = 0;
+    int well_tempered = 0;
+    unsigned long long exceeded = 0;
+
+    int bin_of(double cv) const { return (int)std::floor((cv - cv_min) * bin_width_inverse); }
+    double cv_of(int i) const { return cv_min + i * bin_width; }
+    // UpdatePotential (metadynamics.hpp:103-124)
+    void deposit(double cv)
+    {
+        if (cv < cv_min || cv >= cv_max) exceeded += 1;
+        const double wis = 1.0 / (width * width);
+        for (size_t b = 0; b < grid.size(); ++b) {
+            const double dist = cv - cv_of((int)b);
+            if (well_tempered)
+                grid[b] += weight * std::exp(-0.5 * wis * dist * dist - grid[b] / wt_param);
+            else
+                grid[b] += weight * std::exp(-0.5 * wis * dist * dist);
+        }
+    }
+    // ReturnPotential (metadynamics.hpp:151-179)
+    double potential(double cv) const
+    {
+        const int i = bin_of(cv);
+        if ((unsigned)i < (unsigned)(bins)) {
+            const double ic = (cv - cv_of(i)) * bin_width_inverse;
+            return grid[i] * (1.0 - ic) + ic * grid[i + 1];
+        }
+        if (i < 0) {
+            const double d = cv - cv_min;
+            return grid.front() + threshold * d * d;
+        }
+        const double d = cv - cv_max;
+        return grid.back() + threshold * d * d;
+    }
+    // ReturnDerivative (metadynamics.hpp:181-187)
+    double derivative(double cv) const
+    {
+        const double h = 0.5 * bin_width;
+        return bin_width_inverse * (potential(cv + h) - potential(cv - h));
+    }
 };
 
 struct lt_flow_s {
@@ -111,6 +183,13 @@ inline unsigned map_blocks(const LinkMap& m)
         throw LtError{LT_ESTATE, "invalid thread-block tile for the link kernels"};
     return (unsigned)(m.N[0] * m.N[1] * m.N[2] * m.N[3]);
 }
+
+// The momentum / field update sequence of Integrators::HMC::{Leapfrog, OMF_2, OMF_4}
+// (LettuceGauge/updates/hmc_gauge.hpp:48-75,155-190,241-295).  P(eps, sub) is a
+// momentum update, sub marks the updates whose CV the MetaD path update samples.
+void hmc_schedule(int kind, double tau, int n_step, const std::function<void(double, bool)>& P,
+                  const std::function<void(double)>& U);
+inline bool valid_integrator(int k) { return k == LT_INT_LEAPFROG || k == LT_INT_OMF2 || k == LT_INT_OMF4; }
 
 Geom make_geom(int nt, int nx, int ny, int nz);
 // tile map with extents (tt, tx, ty, tz) over (t, x, y, z/2); returns false if they do not divide

@@ -355,3 +355,44 @@ def metad_omf2(U, P, shape, beta, tau, n_step, rho, n_levels, bias):
     cv_new = cv_charge(U, shape, rho, n_levels)
     H_new = hamiltonian(U, P, shape, beta) + bias.potential(cv_new)
     return H_old, H_new, cv_old, cv_new, samples
+
+
+def uniform(seed, site, stream, counter, k=0, sub=0, mu=0):
+    """First uniform of the Philox draw with counter (site, stream<<24 | k<<4 | sub<<2 | mu, counter)
+    and key seed: the stream-3 accept draw is uniform(seed, 0xFFFFFFFF, 3, counter)."""
+    o = philox([site & 0xFFFFFFFF, (stream << 24) | (k << 4) | (sub << 2) | mu, counter & 0xFFFFFFFF,
+                (counter >> 32) & 0xFFFFFFFF], [seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF])
+    return lib().lto_u01(o[0], o[1])
+
+
+def tempering_accept(bias, cv_unbiased, cv_biased, u):
+    """MetadynamicsTemperingKernel::operator() (LettuceGauge/updates/tempering.hpp:57-72):
+    CV_old is the biased stream's current CV, CV_new the unbiased stream's CV,
+    delta_V = V(CV_new) - V(CV_old), swap if u <= exp(-delta_V)."""
+    delta_v = bias.potential(cv_unbiased) - bias.potential(cv_biased)
+    return u <= np.exp(-delta_v)
+
+
+def ladder_exchange(labels, strengths, cvs, shapes, bias, pt_seed, pt_counter, mode):
+    """Replica exchange over a ladder of bias strengths s_0 < ... < s_{n-1} (the
+    library's generalisation of tempering.hpp:57-72 to n replicas): for adjacent
+    labels k, k+1 held by replicas i, j of equal extents, swap labels if
+    u <= exp(-(s_k - s_{k+1}) (V(Q_j) - V(Q_i))), u = uniform(pt_seed, k, 4, pt_counter).
+    mode 1: k = 0, 2, ...; 2: k = 1, 3, ...; 3: alternate by pt_counter parity.
+    Returns the new labels and the list of (k, accepted)."""
+    labels = list(labels)
+    n = len(labels)
+    out = []
+    if mode == 0:
+        return labels, out
+    first = (pt_counter & 1) if mode == 3 else mode - 1
+    for k in range(first, n - 1, 2):
+        i, j = labels.index(k), labels.index(k + 1)
+        if tuple(shapes[i]) != tuple(shapes[j]):
+            continue
+        delta = (strengths[k] - strengths[k + 1]) * (bias.potential(cvs[j]) - bias.potential(cvs[i]))
+        acc = uniform(pt_seed, k, 4, pt_counter) <= np.exp(-delta)
+        out.append((k, bool(acc)))
+        if acc:
+            labels[i], labels[j] = k + 1, k
+    return labels, out
