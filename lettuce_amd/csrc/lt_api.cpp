@@ -608,7 +608,11 @@ int lt_flow_create(lt_field unflowed, lt_field flowed, int integrator, double ep
         fl->integrator = integrator;
         fl->eps = eps;
         try {
-            fl->z = alloc_links(flowed->g);
+            // Z is an su(3) field: 8 reals per link (4 complex doubles)
+            if (hipMalloc(&fl->z, (size_t)32 * flowed->g.Vh * sizeof(double2)) != hipSuccess) {
+                fl->z = nullptr;
+                throw lt::LtError{LT_ENOMEM, "flow Z allocation failed"};
+            }
             fl->tmp = alloc_links(flowed->g);
         } catch (...) {
             if (fl->z) (void)hipFree(fl->z);

@@ -426,6 +426,56 @@ __device__ __forceinline__ void st_link(double2* __restrict__ U, const Geom& g, 
 #pragma unroll
     for (int c = 0; c < 9; ++c) p[(size_t)c * g.Vh] = make_double2(a.m[c].re, a.m[c].im);
 }
+// su(3) algebra fields (anti-hermitian, traceless: the Wilson-flow Z) in 8 reals,
+//   Z[((mu*2 + parity)*4 + c)*Vh + cb],  c: 0 = (Im a00, Im a11), 1 = a01, 2 = a02, 3 = a12;
+// Re a_ii = 0, a_22 = -(a_00 + a_11) and a_ji = -conj(a_ij) are implied.
+struct Alg8 {
+    double d0, d1;
+    cd a01, a02, a12;
+};
+__device__ __forceinline__ Alg8 to_alg8(const M3& a)
+{
+    return Alg8{a.m[0].im, a.m[4].im, a.m[1], a.m[2], a.m[5]};
+}
+__device__ __forceinline__ M3 from_alg8(const Alg8& z)
+{
+    M3 r;
+    r.m[0] = cd{0.0, z.d0};
+    r.m[4] = cd{0.0, z.d1};
+    r.m[8] = cd{0.0, -(z.d0 + z.d1)};
+    r.m[1] = z.a01;
+    r.m[2] = z.a02;
+    r.m[5] = z.a12;
+    r.m[3] = cd{-z.a01.re, z.a01.im};
+    r.m[6] = cd{-z.a02.re, z.a02.im};
+    r.m[7] = cd{-z.a12.re, z.a12.im};
+    return r;
+}
+__device__ __forceinline__ Alg8 scale(double a, const Alg8& x)
+{
+    return Alg8{a * x.d0, a * x.d1, a * x.a01, a * x.a02, a * x.a12};
+}
+// a * x + b * y
+__device__ __forceinline__ Alg8 axpby(double a, const Alg8& x, double b, const Alg8& y)
+{
+    return Alg8{a * x.d0 + b * y.d0, a * x.d1 + b * y.d1, a * x.a01 + b * y.a01, a * x.a02 + b * y.a02,
+                a * x.a12 + b * y.a12};
+}
+__device__ __forceinline__ Alg8 ld_alg8(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
+{
+    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    const double2 v0 = p[0], v1 = p[(size_t)g.Vh], v2 = p[(size_t)2 * g.Vh], v3 = p[(size_t)3 * g.Vh];
+    return Alg8{v0.x, v0.y, cd{v1.x, v1.y}, cd{v2.x, v2.y}, cd{v3.x, v3.y}};
+}
+__device__ __forceinline__ void st_alg8(double2* __restrict__ Z, const Geom& g, int s, int mu, int par, const Alg8& z)
+{
+    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    p[0] = make_double2(z.d0, z.d1);
+    p[(size_t)g.Vh] = make_double2(z.a01.re, z.a01.im);
+    p[(size_t)2 * g.Vh] = make_double2(z.a02.re, z.a02.im);
+    p[(size_t)3 * g.Vh] = make_double2(z.a12.re, z.a12.im);
+}
+
 // SU(3) links are stored as rows 0 and 1 only: every update ends with the
 // Gram-Schmidt step whose last row is conj(row0 x row1), so row 2 is rebuilt on
 // load with exactly that formula and never travels through memory.

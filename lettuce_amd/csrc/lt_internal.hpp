@@ -143,7 +143,7 @@ struct lt_flow_s {
     lt_field flowed = nullptr;
     int integrator = LT_FLOW_RK3;
     double eps = 0.0;
-    double2* z = nullptr;      // force / RK3 Z field, same layout as a link field
+    double2* z = nullptr;      // RK3 Z field: su(3), 8 reals per link (ld_alg8 / st_alg8 layout)
     double2* tmp = nullptr;    // second link buffer for the Jacobi stages
 };
 

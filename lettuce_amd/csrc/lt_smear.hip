@@ -33,17 +33,10 @@ __global__ void LT_LINK_KERNEL flow_stage_kernel(const double2* __restrict__ Uin
     if (!map_link_all(g, m, mu, par, c, s)) return;
     M3 st = staple(Uin, g, s, c, mu);
     M3 L = ldu(Uin, g, s, mu, par);
-    M3 F = algebra(mul_nd(st, L));
-    M3 z;
-    if (read_z) {
-        M3 zo = ld(Z, g, s, mu, par);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) z.m[k] = c_old * zo.m[k] + c_new * F.m[k];
-    } else {
-        z = scale(c_new, F);
-    }
-    if (write_z) st_link(Z, g, s, mu, par, z);
-    M3 W = mul(exp_ah(z, c_exp), L);
+    const Alg8 F = to_alg8(algebra(mul_nd(st, L)));
+    const Alg8 z = read_z ? axpby(c_old, ld_alg8(Z, g, s, mu, par), c_new, F) : scale(c_new, F);
+    if (write_z) st_alg8(Z, g, s, mu, par, z);
+    M3 W = mul(exp_ah(from_alg8(z), c_exp), L);
     gram_schmidt(W);
     stu(Uout, g, s, mu, par, W);
 }
