@@ -369,6 +369,7 @@ class CollectiveVariable:
         h = ctypes.c_void_p()
         call("lt_cv_create", ctx.handle, *shape, n_levels, rho, ctypes.byref(h))
         self._h = h
+        self.ctx = ctx      # the context must outlive the workspace
         self.shape = tuple(shape)
 
     @property
@@ -484,16 +485,26 @@ class HMCMetaDKernel:
 
 
 class _ReplicaField(GaugeField):
-    """Gauge field of one replica of a ReplicaBatch; the batch owns its memory."""
+    """Gauge field of replica r of a ReplicaBatch; the batch owns its memory and
+    the view keeps the batch alive.  After the batch is closed the view raises."""
 
-    def __init__(self, ctx, shape, handle, batch):
-        self.ctx = ctx
-        self.shape = tuple(shape)
-        self._h = handle
-        self._batch = batch   # keeps the batch alive while the view is used
+    def __init__(self, batch, r):
+        self.ctx = batch.ctx
+        self.shape = batch.shapes[r]
+        self._batch = batch
+        self._r = r
+
+    @property
+    def _h(self):
+        bh = self._batch.handle
+        if bh is None or not bh.value:
+            raise LettuceError("lt_batch_field", -4, "the batch of this replica was closed")
+        f = ctypes.c_void_p()
+        call("lt_batch_field", bh, self._r, ctypes.byref(f))
+        return f
 
     def close(self):
-        self._h = None
+        pass
 
 
 SWAP_MODES = {"none": 0, "even": 1, "odd": 2, "alternate": 3}
@@ -519,18 +530,15 @@ class ReplicaBatch:
         self._h = h
         self.n = len(self.shapes)
         self._n_sub = 0
-        self._views = []
-        for r, s in enumerate(self.shapes):
-            f = ctypes.c_void_p()
-            call("lt_batch_field", h, r, ctypes.byref(f))
-            self._views.append(_ReplicaField(ctx, s, f, self))
 
     @property
     def handle(self):
         return self._h
 
     def field(self, r):
-        return self._views[r]
+        if not 0 <= r < self.n:
+            raise IndexError("replica index out of range")
+        return _ReplicaField(self, r)
 
     def set_rng(self, r, seed, counter=0):
         call("lt_batch_set_rng", self._h, r, seed, counter)
@@ -585,8 +593,6 @@ class ReplicaBatch:
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
-            for v in self._views:
-                v._h = None
             call("lt_batch_destroy", self._h)
             self._h = None
 

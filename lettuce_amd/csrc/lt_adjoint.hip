@@ -6,7 +6,7 @@
 // Buffers (general 3x3 layout unless noted):
 //   levels    n+1 link fields (two-row SU(3))
 //   tape      per level, per link: the hermitian exponent Q = -i rho Algebra(st U^+)
-//   clover    6 planes per site: C[((plane*2 + parity)*9 + c)*Vh + cb]
+//   clover    6 planes per site, R = C - C^+ in 9 reals: C[((plane*2 + parity)*5 + k)*Vh + cb]
 //   sigma     force being propagated (in place, level by level)
 //   lambda    per-link Lambda of the current level
 // The per-thread bodies live in lt_bodies.hpp and are shared with the
@@ -61,10 +61,11 @@ __global__ void __launch_bounds__(CBS) sum_kernel(const double* part, int n, dou
     if (threadIdx.x == 0) *out = sh[0];
 }
 
+template <int HALF, int K>
 __global__ void LT_LINK_KERNEL clover_deriv_kernel(const double2* __restrict__ U, const double2* __restrict__ C,
                                                    double2* __restrict__ F, Geom g, LinkMap m, double scale_factor)
 {
-    body_clover_deriv(U, C, F, g, m, xcd_block(blockIdx.x, gridDim.x), scale_factor);
+    body_clover_deriv<HALF, K>(U, C, F, g, m, xcd_block(blockIdx.x, gridDim.x), scale_factor);
 }
 
 __global__ void LT_LINK_KERNEL lambda_kernel(const double2* __restrict__ U, const double2* __restrict__ Uprev,
@@ -104,8 +105,13 @@ double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C)
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
                     double scale_factor)
 {
-    launch(ctx, "clover_deriv", clover_deriv_kernel, dim3(8 * map_blocks(m)), dim3(m.nthreads), 0, U, C, F, g, m,
-           scale_factor);
+    const dim3 grid(8 * map_blocks(m)), block(m.nthreads);
+    launch(ctx, "clover_deriv", clover_deriv_kernel<0, 0>, grid, block, 0, U, C, F, g, m, scale_factor);
+    launch(ctx, "clover_deriv", clover_deriv_kernel<1, 0>, grid, block, 0, U, C, F, g, m, scale_factor);
+    launch(ctx, "clover_deriv", clover_deriv_kernel<0, 1>, grid, block, 0, U, C, F, g, m, scale_factor);
+    launch(ctx, "clover_deriv", clover_deriv_kernel<1, 1>, grid, block, 0, U, C, F, g, m, scale_factor);
+    launch(ctx, "clover_deriv", clover_deriv_kernel<0, 2>, grid, block, 0, U, C, F, g, m, scale_factor);
+    launch(ctx, "clover_deriv", clover_deriv_kernel<1, 2>, grid, block, 0, U, C, F, g, m, scale_factor);
 }
 
 void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,

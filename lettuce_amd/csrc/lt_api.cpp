@@ -202,6 +202,9 @@ int lt_ctx_create(int device, lt_ctx* out)
 int lt_ctx_destroy(lt_ctx ctx)
 {
     if (!ctx) return LT_OK;
+    if (ctx->live > 0)
+        return fail(LT_ESTATE, "lt_ctx_destroy: " + std::to_string(ctx->live) +
+                                   " field / flow / workspace / batch handle(s) of this context still exist");
     return guarded([&] {
         use_device(ctx);
         LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -275,6 +278,7 @@ int lt_field_create(lt_ctx ctx, int nt, int nx, int ny, int nz, lt_field* out)
             delete f;
             throw;
         }
+        ctx->live += 1;
         *out = f;
         return LT_OK;
     });
@@ -288,6 +292,7 @@ int lt_field_destroy(lt_field f)
         use_device(f->ctx);
         LT_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
         if (f->d) LT_HIP_CHECK(hipFree(f->d));
+        f->ctx->live -= 1;
         delete f;
         return LT_OK;
     });
@@ -619,6 +624,7 @@ int lt_flow_create(lt_field unflowed, lt_field flowed, int integrator, double ep
             delete fl;
             throw;
         }
+        flowed->ctx->live += 1;
         *out = fl;
         return LT_OK;
     });
@@ -632,6 +638,7 @@ int lt_flow_destroy(lt_flow fl)
         LT_HIP_CHECK(hipStreamSynchronize(fl->flowed->ctx->stream));
         (void)hipFree(fl->z);
         (void)hipFree(fl->tmp);
+        fl->flowed->ctx->live -= 1;
         delete fl;
         return LT_OK;
     });

@@ -62,11 +62,12 @@ __global__ void LT_LINK_KERNEL bk_stout_tape(const double2* __restrict__ Uin, do
     body_stout_tape(Uin + off, Uout + off, T + off, g, m, lb, rho);
 }
 
+template <int HALF, int K>
 __global__ void LT_LINK_KERNEL bk_clover_deriv(const double2* __restrict__ U, const double2* __restrict__ C,
                                                double2* __restrict__ F, BatchGrid bg, const double* __restrict__ scale)
 {
     LT_BATCH_LINK_PROLOGUE
-    body_clover_deriv(U + off, C + off + off / 2, F + off, g, m, lb, scale[r]);
+    body_clover_deriv<HALF, K>(U + off, C + clover_offset(off), F + off, g, m, lb, scale[r]);
 }
 
 __global__ void LT_LINK_KERNEL bk_lambda(const double2* __restrict__ U, const double2* __restrict__ Uprev,
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(RBS) bk_clover_store(const double2* __restrict
     const Geom g = bg.reps[r].g;
     const long long off = bg.reps[r].off;
     const int s = (blockIdx.x - bg.sb[r]) * RBS + threadIdx.x;
-    if (s < g.V) body_clover_store(U + off, C + off + off / 2, g, s, blockIdx.y);
+    if (s < g.V) body_clover_store(U + off, C + clover_offset(off), g, s, blockIdx.y);
 }
 
 __global__ void __launch_bounds__(RBS) bk_qdensity(const double2* __restrict__ C, BatchGrid bg, double* part)
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(RBS) bk_qdensity(const double2* __restrict__ C
     const Geom g = bg.reps[r].g;
     const long long off = bg.reps[r].off;
     const int s = (blockIdx.x - bg.sb[r]) * RBS + threadIdx.x;
-    block_sum(s < g.V ? body_qdensity_site(C + off + off / 2, g, s) : 0.0, part + blockIdx.x);
+    block_sum(s < g.V ? body_qdensity_site(C + clover_offset(off), g, s) : 0.0, part + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(RBS) bk_plaq(const double2* __restrict__ U, BatchGrid bg, double* part)
@@ -382,8 +383,13 @@ void kb_clover_charge(lt_ctx ctx, const BatchGrid& bg, int total_sb, const doubl
 void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* C, double2* F,
                      const double* scale)
 {
-    launch(ctx, "batch_clover_deriv", bk_clover_deriv, dim3(total_lb), dim3(link_threads(bg, total_lb)), 0, U, C, F,
-           bg, scale);
+    const dim3 grid(total_lb), block(link_threads(bg, total_lb));
+    launch(ctx, "batch_clover_deriv", bk_clover_deriv<0, 0>, grid, block, 0, U, C, F, bg, scale);
+    launch(ctx, "batch_clover_deriv", bk_clover_deriv<1, 0>, grid, block, 0, U, C, F, bg, scale);
+    launch(ctx, "batch_clover_deriv", bk_clover_deriv<0, 1>, grid, block, 0, U, C, F, bg, scale);
+    launch(ctx, "batch_clover_deriv", bk_clover_deriv<1, 1>, grid, block, 0, U, C, F, bg, scale);
+    launch(ctx, "batch_clover_deriv", bk_clover_deriv<0, 2>, grid, block, 0, U, C, F, bg, scale);
+    launch(ctx, "batch_clover_deriv", bk_clover_deriv<1, 2>, grid, block, 0, U, C, F, bg, scale);
 }
 
 void kb_stout_recursion(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* Uprev,

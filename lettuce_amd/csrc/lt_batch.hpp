@@ -9,7 +9,7 @@
 namespace lt {
 
 // One replica of a batch.  Link-layout buffers of replica r start at off
-// (in complex doubles, 72*Vh per field); the six-plane clover buffer at 3*off/2.
+// (in complex doubles, 72*Vh per field); the clover planes at clover_offset(off).
 struct BatchRep {
     Geom g;
     LinkMap m;

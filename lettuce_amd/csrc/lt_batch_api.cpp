@@ -245,7 +245,7 @@ int lt_batch_create(lt_ctx ctx, int n_rep, const int* shapes, int n_levels, doub
             b->P = dev_alloc<double2>(nl);
             b->F = dev_alloc<double2>(nl);
             b->lambda = dev_alloc<double2>(nl);
-            b->clover = dev_alloc<double2>(nl + nl / 2);
+            b->clover = dev_alloc<double2>((size_t)lt::clover_offset(b->total));
             b->levels.push_back(b->U);
             for (int l = 0; l < n_levels; ++l) {
                 b->levels.push_back(dev_alloc<double2>(nl));
@@ -313,6 +313,7 @@ int lt_batch_create(lt_ctx ctx, int n_rep, const int* shapes, int n_levels, doub
             delete b;
             throw;
         }
+        ctx->live += 1;
         *out = b;
         return LT_OK;
     });
@@ -325,6 +326,7 @@ int lt_batch_destroy(lt_batch b)
         LT_HIP_CHECK(hipSetDevice(b->ctx->device));
         LT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
         b->release();
+        b->ctx->live -= 1;
         delete b;
         return LT_OK;
     });

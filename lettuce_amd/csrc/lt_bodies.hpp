@@ -110,34 +110,45 @@ __device__ __forceinline__ int plane_of(int a, int b)
     return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5);
 }
 
-__device__ __forceinline__ M3 ld_plane(const double2* __restrict__ C, const Geom& g, int s, int plane, int par)
+// The clover planes are kept as R_munu = C_munu - C_munu^+ (mu < nu), which is all
+// the charge and its derivative use.  R is anti-hermitian: 9 reals stored exactly,
+//   R[((plane*2 + parity)*5 + k)*Vh + cb],
+//   k: 0 r01, 1 r02, 2 r12, 3 (Im r00, Im r11), 4 (Im r22, 0);
+// Re r_ii = 0 and r_ji = -conj(r_ij) hold exactly for C - C^+ in floating point.
+__device__ __forceinline__ M3 ld_plane(const double2* __restrict__ R, const Geom& g, int s, int plane, int par)
 {
-    const double2* p = C + (size_t)((plane * 2 + par) * 9) * g.Vh + (s >> 1);
+    const double2* p = R + (size_t)((plane * 2 + par) * R_SLOTS) * g.Vh + (s >> 1);
+    const double2 a01 = p[0], a02 = p[(size_t)g.Vh], a12 = p[(size_t)2 * g.Vh];
+    const double2 d01 = p[(size_t)3 * g.Vh], d2 = p[(size_t)4 * g.Vh];
     M3 r;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        double2 v = p[(size_t)k * g.Vh];
-        r.m[k] = cd{v.x, v.y};
-    }
+    r.m[0] = cd{0.0, d01.x};
+    r.m[4] = cd{0.0, d01.y};
+    r.m[8] = cd{0.0, d2.x};
+    r.m[1] = cd{a01.x, a01.y};
+    r.m[2] = cd{a02.x, a02.y};
+    r.m[5] = cd{a12.x, a12.y};
+    r.m[3] = cd{-a01.x, a01.y};
+    r.m[6] = cd{-a02.x, a02.y};
+    r.m[7] = cd{-a12.x, a12.y};
     return r;
 }
-__device__ __forceinline__ void st_plane(double2* __restrict__ C, const Geom& g, int s, int plane, int par, const M3& a)
+// stores R = C - C^+ of the clover term C
+__device__ __forceinline__ void st_plane(double2* __restrict__ R, const Geom& g, int s, int plane, int par, const M3& c)
 {
-    double2* p = C + (size_t)((plane * 2 + par) * 9) * g.Vh + (s >> 1);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) p[(size_t)k * g.Vh] = make_double2(a.m[k].re, a.m[k].im);
+    double2* p = R + (size_t)((plane * 2 + par) * R_SLOTS) * g.Vh + (s >> 1);
+    const cd r01 = c.m[1] - conj(c.m[3]), r02 = c.m[2] - conj(c.m[6]), r12 = c.m[5] - conj(c.m[7]);
+    p[0] = make_double2(r01.re, r01.im);
+    p[(size_t)g.Vh] = make_double2(r02.re, r02.im);
+    p[(size_t)2 * g.Vh] = make_double2(r12.re, r12.im);
+    p[(size_t)3 * g.Vh] = make_double2(c.m[0].im + c.m[0].im, c.m[4].im + c.m[4].im);
+    p[(size_t)4 * g.Vh] = make_double2(c.m[8].im + c.m[8].im, 0.0);
 }
 
 // R_rhosigma(y) = C_rhosigma - C_sigmarho with C_sigmarho = C_rhosigma^+
 __device__ __forceinline__ M3 R_at(const double2* __restrict__ C, const Geom& g, int y, int ypar, int rho, int sigma)
 {
     const int lo = rho < sigma ? rho : sigma, hi = rho < sigma ? sigma : rho;
-    M3 c = ld_plane(C, g, y, plane_of(lo, hi), ypar);
-    M3 r;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) r.m[3 * i + j] = c.m[3 * i + j] - conj(c.m[3 * j + i]);
+    M3 r = ld_plane(C, g, y, plane_of(lo, hi), ypar);
     return rho < sigma ? r : scale(-1.0, r);
 }
 
@@ -191,73 +202,111 @@ __device__ __forceinline__ double body_qdensity_site(const double2* __restrict__
          + retr_mul(R_at(C, g, s, par, 0, 3), R_at(C, g, s, par, 1, 2));
 }
 
-// One epsilon-term of the clover-charge derivative for link (x, mu):
-//   tmp1 (U_nu(x)^+ R(x) + R(x+nu) U_nu(x)^+) + (U_nu(x+mu) R(x+mu+nu) + R(x+mu) U_nu(x+mu)) tmp2
-//   - tmp3 (U_nu(x-nu) R(x) + R(x-nu) U_nu(x-nu)) - (U_nu(x+mu-nu)^+ R(x+mu-nu) + R(x+mu) U_nu(x+mu-nu)^+) tmp4
+// Clover-charge derivative for link (x, mu) (CloverDerivativeComponent,
+// LettuceGauge/observables/clover.hpp:238-321): for each of the three nu != mu,
+// with (rho, sigma) the remaining directions and s = eps(mu, nu, rho, sigma),
+//   upper: tmp1 (U_nu(x)^+ R(x) + R(x+nu) U_nu(x)^+) + (U_nu(x+mu) R(x+mu+nu) + R(x+mu) U_nu(x+mu)) tmp2
+//   lower: tmp3 (U_nu(x-nu) R(x) + R(x-nu) U_nu(x-nu)) + (U_nu(x+mu-nu)^+ R(x+mu-nu) + R(x+mu) U_nu(x+mu-nu)^+) tmp4
 // with tmp1 = U_nu(x+mu) U_mu(x+nu)^+, tmp2 = U_mu(x+nu)^+ U_nu(x)^+,
-//      tmp3 = U_nu(x+mu-nu)^+ U_mu(x-nu)^+, tmp4 = U_mu(x-nu)^+ U_nu(x-nu).
-__device__ __forceinline__ M3 cderiv_term(const double2* __restrict__ U, const double2* __restrict__ C, const Geom& g,
-                                          int s, const int c[4], int par, int mu, int nu, int rho, int sigma)
+//      tmp3 = U_nu(x+mu-nu)^+ U_mu(x-nu)^+, tmp4 = U_mu(x-nu)^+ U_nu(x-nu),
+// accumulate sum_nu s (upper - lower) and end with
+//   F(x, mu) = scale * (-2/(512 pi^2)) Algebra(U_mu(x) * sum).
+// One half-term alone needs ~210 VGPRs; the whole sum in one thread spills, so the
+// derivative runs as six launches (term k = 0..2, upper then lower) that
+// accumulate in F, the last one applying the final projection.
+
+// nu = k-th direction != mu, (rho, sigma) the remaining two in ascending order,
+// sign = eps(mu, nu, rho, sigma)
+template <int MU, int K>
+struct CloverTerm {
+    static constexpr int nu = K < MU ? K : K + 1;
+    static constexpr int rho = (MU != 0 && nu != 0) ? 0 : ((MU != 1 && nu != 1) ? 1 : 2);
+    static constexpr int sigma = (MU != 3 && nu != 3) ? 3 : ((MU != 2 && nu != 2) ? 2 : 1);
+    static constexpr int inversions = (MU > nu) + (MU > rho) + (MU > sigma) + (nu > rho) + (nu > sigma) + (rho > sigma);
+    static constexpr double sign = (inversions & 1) ? -1.0 : 1.0;
+};
+
+template <int MU, int NU, int RHO, int SIGMA>
+__device__ __forceinline__ M3 cderiv_upper(const double2* __restrict__ U, const double2* __restrict__ C, const Geom& g,
+                                           int s, const int c[4], int par)
 {
     const int q = par ^ 1;
-    const int smu = up(g, s, c[mu], mu), snu = up(g, s, c[nu], nu), snd = dn(g, s, c[nu], nu);
-    const int smu_nu = up(g, smu, c[nu], nu), smu_nd = dn(g, smu, c[nu], nu);
-    const M3 Unu_x = ldu(U, g, s, nu, par), Unu_mu = ldu(U, g, smu, nu, q), Umu_nu = ldu(U, g, snu, mu, q);
-    const M3 Rmu = R_at(C, g, smu, q, rho, sigma);
-    M3 acc = mul(mul_nd(Unu_mu, Umu_nu), add(mul_dn(Unu_x, R_at(C, g, s, par, rho, sigma)),
-                                               mul_nd(R_at(C, g, snu, q, rho, sigma), Unu_x)));
+    const int smu = up(g, s, c[MU], MU), snu = up(g, s, c[NU], NU);
+    const M3 Unu_x = ldu(U, g, s, NU, par);
+    M3 tmp1, tmp2;
     {
-        M3 a = add(mul(Unu_mu, R_at(C, g, smu_nu, par, rho, sigma)), mul(Rmu, Unu_mu));
-        M3 tmp2 = mul_nd(dag(Umu_nu), Unu_x);
-        acc = add(acc, mul(a, tmp2));
+        const M3 Umu_nu = ldu(U, g, snu, MU, q);
+        tmp2 = mul_nd(dag(Umu_nu), Unu_x);
+        tmp1 = mul_nd(ldu(U, g, smu, NU, q), Umu_nu);
     }
-    const M3 Unu_mund = ldu(U, g, smu_nd, nu, par), Umu_nd = ldu(U, g, snd, mu, q), Unu_nd = ldu(U, g, snd, nu, q);
-    {
-        M3 tmp3 = mul_nd(dag(Unu_mund), Umu_nd);
-        M3 bb = add(mul(Unu_nd, R_at(C, g, s, par, rho, sigma)), mul(R_at(C, g, snd, q, rho, sigma), Unu_nd));
-        acc = sub(acc, mul(tmp3, bb));
-    }
-    {
-        M3 tmp4 = mul_dn(Umu_nd, Unu_nd);
-        M3 e = add(mul_dn(Unu_mund, R_at(C, g, smu_nd, par, rho, sigma)), mul_nd(Rmu, Unu_mund));
-        acc = sub(acc, mul(e, tmp4));
-    }
-    return acc;
+    M3 S = mul_dn(Unu_x, R_at(C, g, s, par, RHO, SIGMA));
+    S = add(S, mul_nd(R_at(C, g, snu, q, RHO, SIGMA), Unu_x));
+    M3 acc = mul(tmp1, S);
+    const M3 Unu_mu = ldu(U, g, smu, NU, q);
+    S = mul(Unu_mu, R_at(C, g, up(g, smu, c[NU], NU), par, RHO, SIGMA));
+    S = add(S, mul(R_at(C, g, smu, q, RHO, SIGMA), Unu_mu));
+    return add(acc, mul(S, tmp2));
 }
 
-// F(x, mu) = scale * (-2/(512 pi^2)) Algebra(U_mu(x) * sum of the three epsilon terms)
+template <int MU, int NU, int RHO, int SIGMA>
+__device__ __forceinline__ M3 cderiv_lower(const double2* __restrict__ U, const double2* __restrict__ C, const Geom& g,
+                                           int s, const int c[4], int par)
+{
+    const int q = par ^ 1;
+    const int smu = up(g, s, c[MU], MU);
+    const int snd = dn(g, s, c[NU], NU), smu_nd = dn(g, smu, c[NU], NU);
+    const M3 Unu_nd = ldu(U, g, snd, NU, q);
+    const M3 Unu_mund = ldu(U, g, smu_nd, NU, par);
+    M3 tmp3, tmp4;
+    {
+        const M3 Umu_nd = ldu(U, g, snd, MU, q);
+        tmp3 = mul_nd(dag(Unu_mund), Umu_nd);
+        tmp4 = mul_dn(Umu_nd, Unu_nd);
+    }
+    M3 S = mul(Unu_nd, R_at(C, g, s, par, RHO, SIGMA));
+    S = add(S, mul(R_at(C, g, snd, q, RHO, SIGMA), Unu_nd));
+    const M3 acc = mul(tmp3, S);
+    S = mul_dn(Unu_mund, R_at(C, g, smu_nd, par, RHO, SIGMA));
+    S = add(S, mul_nd(R_at(C, g, smu, q, RHO, SIGMA), Unu_mund));
+    return add(acc, mul(S, tmp4));
+}
+
+template <int HALF, int K, int MU>
+__device__ __forceinline__ void cderiv_half_mu(const double2* __restrict__ U, const double2* __restrict__ C,
+                                               double2* __restrict__ F, const Geom& g, int s, const int c[4], int par,
+                                               double scale_factor)
+{
+    using T = CloverTerm<MU, K>;
+    M3 acc;
+    if (HALF == 0) {
+        const M3 up_part = scale(T::sign, cderiv_upper<MU, T::nu, T::rho, T::sigma>(U, C, g, s, c, par));
+        acc = K == 0 ? up_part : add(ld(F, g, s, MU, par), up_part);
+        st_link(F, g, s, MU, par, acc);
+    } else {
+        const M3 low = scale(T::sign, cderiv_lower<MU, T::nu, T::rho, T::sigma>(U, C, g, s, c, par));
+        acc = sub(ld(F, g, s, MU, par), low);
+        if (K == 2) {
+            const double pref = -2.0 / (512.0 * 3.14159265358979323846 * 3.14159265358979323846);
+            acc = scale(scale_factor * pref, algebra(mul(ldu(U, g, s, MU, par), acc)));
+        }
+        st_link(F, g, s, MU, par, acc);
+    }
+}
+
+// one of the six launches: half HALF (0 upper, 1 lower) of term K
+template <int HALF, int K>
 __device__ __forceinline__ void body_clover_deriv(const double2* __restrict__ U, const double2* __restrict__ C,
                                                   double2* __restrict__ F, const Geom& g, const LinkMap& m, int b,
                                                   double scale_factor)
 {
     int mu, par, c[4], s;
     if (!map_link_b(g, m, b, mu, par, c, s)) return;
-    M3 acc;
     switch (mu) {
-        case 0:
-            acc = cderiv_term(U, C, g, s, c, par, 0, 1, 2, 3);
-            acc = sub(acc, cderiv_term(U, C, g, s, c, par, 0, 2, 1, 3));
-            acc = add(acc, cderiv_term(U, C, g, s, c, par, 0, 3, 1, 2));
-            break;
-        case 1:
-            acc = scale(-1.0, cderiv_term(U, C, g, s, c, par, 1, 0, 2, 3));
-            acc = add(acc, cderiv_term(U, C, g, s, c, par, 1, 2, 0, 3));
-            acc = sub(acc, cderiv_term(U, C, g, s, c, par, 1, 3, 0, 2));
-            break;
-        case 2:
-            acc = cderiv_term(U, C, g, s, c, par, 2, 0, 1, 3);
-            acc = sub(acc, cderiv_term(U, C, g, s, c, par, 2, 1, 0, 3));
-            acc = add(acc, cderiv_term(U, C, g, s, c, par, 2, 3, 0, 1));
-            break;
-        default:
-            acc = scale(-1.0, cderiv_term(U, C, g, s, c, par, 3, 0, 1, 2));
-            acc = add(acc, cderiv_term(U, C, g, s, c, par, 3, 1, 0, 2));
-            acc = sub(acc, cderiv_term(U, C, g, s, c, par, 3, 2, 0, 1));
-            break;
+        case 0: cderiv_half_mu<HALF, K, 0>(U, C, F, g, s, c, par, scale_factor); break;
+        case 1: cderiv_half_mu<HALF, K, 1>(U, C, F, g, s, c, par, scale_factor); break;
+        case 2: cderiv_half_mu<HALF, K, 2>(U, C, F, g, s, c, par, scale_factor); break;
+        default: cderiv_half_mu<HALF, K, 3>(U, C, F, g, s, c, par, scale_factor); break;
     }
-    acc = mul(ldu(U, g, s, mu, par), acc);
-    const double pref = -2.0 / (512.0 * 3.14159265358979323846 * 3.14159265358979323846);
-    st_link(F, g, s, mu, par, scale(scale_factor * pref, algebra(acc)));
 }
 
 // Sigma' = Uprev^+ Sigma (stored in place) and

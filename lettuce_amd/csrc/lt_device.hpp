@@ -66,6 +66,20 @@ __device__ __forceinline__ cd cmul(cd a, cd b)
 {
     return cd{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
 }
+// conj(x * y - z * w) as two chains of fused multiply-adds (the third row of an
+// SU(3) matrix from its first two rows)
+__device__ __forceinline__ cd conj_cross(cd x, cd y, cd z, cd w)
+{
+    double re = z.im * w.im;
+    re = fma(-z.re, w.re, re);
+    re = fma(-x.im, y.im, re);
+    re = fma(x.re, y.re, re);
+    double im = z.im * w.re;
+    im = fma(z.re, w.im, im);
+    im = fma(-x.im, y.re, im);
+    im = fma(-x.re, y.im, im);
+    return cd{re, im};
+}
 
 // ---------------------------------------------------------------------------
 // 3x3 complex matrix, row-major
@@ -86,6 +100,77 @@ __device__ __forceinline__ M3 m3_id()
     r.m[0].re = r.m[4].re = r.m[8].re = 1.0;
     return r;
 }
+// sum_k op(a_k) op(b_k) over three complex pairs as two chains of one multiply
+// and five fused multiply-adds (op = conj where CA / CB is set): 12 FP64
+// instructions per complex entry instead of 16 for separate products and sums.
+template <bool CA, bool CB>
+__device__ __forceinline__ cd dot3(cd a0, cd b0, cd a1, cd b1, cd a2, cd b2)
+{
+    // (x.re + i sx x.im)(y.re + i sy y.im): re = x.re y.re - sx sy x.im y.im, im = sy x.re y.im + sx x.im y.re
+    constexpr double sa = CA ? -1.0 : 1.0, sb = CB ? -1.0 : 1.0, sab = -sa * sb;
+    double re = a0.re * b0.re;
+    re = fma(sab * a0.im, b0.im, re);
+    re = fma(a1.re, b1.re, re);
+    re = fma(sab * a1.im, b1.im, re);
+    re = fma(a2.re, b2.re, re);
+    re = fma(sab * a2.im, b2.im, re);
+    double im = (sb * a0.re) * b0.im;
+    im = fma(sa * a0.im, b0.re, im);
+    im = fma(sb * a1.re, b1.im, im);
+    im = fma(sa * a1.im, b1.re, im);
+    im = fma(sb * a2.re, b2.im, im);
+    im = fma(sa * a2.im, b2.re, im);
+    return cd{re, im};
+}
+// acc + sum_k op(a_k) op(b_k): the same chains started from acc (6 FMAs per part)
+template <bool CA, bool CB>
+__device__ __forceinline__ cd dot3_acc(cd acc, cd a0, cd b0, cd a1, cd b1, cd a2, cd b2)
+{
+    constexpr double sa = CA ? -1.0 : 1.0, sb = CB ? -1.0 : 1.0, sab = -sa * sb;
+    double re = fma(a0.re, b0.re, acc.re);
+    re = fma(sab * a0.im, b0.im, re);
+    re = fma(a1.re, b1.re, re);
+    re = fma(sab * a1.im, b1.im, re);
+    re = fma(a2.re, b2.re, re);
+    re = fma(sab * a2.im, b2.im, re);
+    double im = fma(sb * a0.re, b0.im, acc.im);
+    im = fma(sa * a0.im, b0.re, im);
+    im = fma(sb * a1.re, b1.im, im);
+    im = fma(sa * a1.im, b1.re, im);
+    im = fma(sb * a2.re, b2.im, im);
+    im = fma(sa * a2.im, b2.re, im);
+    return cd{re, im};
+}
+// acc += a * b
+__device__ __forceinline__ void mul_acc(M3& acc, const M3& a, const M3& b)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            acc.m[3 * i + j] = dot3_acc<false, false>(acc.m[3 * i + j], a.m[3 * i], b.m[j], a.m[3 * i + 1], b.m[3 + j],
+                                                      a.m[3 * i + 2], b.m[6 + j]);
+}
+// acc += a * b^+
+__device__ __forceinline__ void mul_nd_acc(M3& acc, const M3& a, const M3& b)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            acc.m[3 * i + j] = dot3_acc<false, true>(acc.m[3 * i + j], a.m[3 * i], b.m[3 * j], a.m[3 * i + 1],
+                                                     b.m[3 * j + 1], a.m[3 * i + 2], b.m[3 * j + 2]);
+}
+// acc += a^+ * b
+__device__ __forceinline__ void mul_dn_acc(M3& acc, const M3& a, const M3& b)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            acc.m[3 * i + j] = dot3_acc<true, false>(acc.m[3 * i + j], a.m[i], b.m[j], a.m[3 + i], b.m[3 + j],
+                                                     a.m[6 + i], b.m[6 + j]);
+}
 // a * b
 __device__ __forceinline__ M3 mul(const M3& a, const M3& b)
 {
@@ -94,7 +179,7 @@ __device__ __forceinline__ M3 mul(const M3& a, const M3& b)
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            r.m[3 * i + j] = a.m[3 * i] * b.m[j] + a.m[3 * i + 1] * b.m[3 + j] + a.m[3 * i + 2] * b.m[6 + j];
+            r.m[3 * i + j] = dot3<false, false>(a.m[3 * i], b.m[j], a.m[3 * i + 1], b.m[3 + j], a.m[3 * i + 2], b.m[6 + j]);
     return r;
 }
 // a * b^+
@@ -105,8 +190,8 @@ __device__ __forceinline__ M3 mul_nd(const M3& a, const M3& b)
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            r.m[3 * i + j] = mul_c(a.m[3 * i], b.m[3 * j]) + mul_c(a.m[3 * i + 1], b.m[3 * j + 1])
-                           + mul_c(a.m[3 * i + 2], b.m[3 * j + 2]);
+            r.m[3 * i + j] = dot3<false, true>(a.m[3 * i], b.m[3 * j], a.m[3 * i + 1], b.m[3 * j + 1], a.m[3 * i + 2],
+                                               b.m[3 * j + 2]);
     return r;
 }
 // a^+ * b
@@ -117,7 +202,7 @@ __device__ __forceinline__ M3 mul_dn(const M3& a, const M3& b)
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            r.m[3 * i + j] = cmul(a.m[i], b.m[j]) + cmul(a.m[3 + i], b.m[3 + j]) + cmul(a.m[6 + i], b.m[6 + j]);
+            r.m[3 * i + j] = dot3<true, false>(a.m[i], b.m[j], a.m[3 + i], b.m[3 + j], a.m[6 + i], b.m[6 + j]);
     return r;
 }
 __device__ __forceinline__ M3 dag(const M3& a)
@@ -221,9 +306,9 @@ __device__ __forceinline__ void gram_schmidt(M3& a)
     m[3] = n1 * m[3];
     m[4] = n1 * m[4];
     m[5] = n1 * m[5];
-    m[6] = conj(m[1] * m[5] - m[2] * m[4]);
-    m[7] = conj(m[2] * m[3] - m[0] * m[5]);
-    m[8] = conj(m[0] * m[4] - m[1] * m[3]);
+    m[6] = conj_cross(m[1], m[5], m[2], m[4]);
+    m[7] = conj_cross(m[2], m[3], m[0], m[5]);
+    m[8] = conj_cross(m[0], m[4], m[1], m[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -482,9 +567,9 @@ __device__ __forceinline__ void st_alg8(double2* __restrict__ Z, const Geom& g, 
 __device__ __forceinline__ void third_row(M3& a)
 {
     cd* m = a.m;
-    m[6] = conj(m[1] * m[5] - m[2] * m[4]);
-    m[7] = conj(m[2] * m[3] - m[0] * m[5]);
-    m[8] = conj(m[0] * m[4] - m[1] * m[3]);
+    m[6] = conj_cross(m[1], m[5], m[2], m[4]);
+    m[7] = conj_cross(m[2], m[3], m[0], m[5]);
+    m[8] = conj_cross(m[0], m[4], m[1], m[3]);
 }
 __device__ __forceinline__ M3 ldu(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
@@ -504,6 +589,13 @@ __device__ __forceinline__ void stu(double2* __restrict__ U, const Geom& g, int 
 #pragma unroll
     for (int c = 0; c < 6; ++c) p[(size_t)c * g.Vh] = make_double2(a.m[c].re, a.m[c].im);
 }
+
+// Clover planes R_munu = C_munu - C_munu^+ (mu < nu) are stored anti-hermitian in
+// 9 reals, R_SLOTS complex slots per (plane, parity): 12 * R_SLOTS * Vh complex
+// doubles per lattice; a replica at link offset off (72 * Vh units) has its
+// planes at clover_offset(off).
+constexpr int R_SLOTS = 5;
+__host__ __device__ constexpr long long clover_offset(long long link_off) { return link_off / 72 * (12 * R_SLOTS); }
 
 __device__ __forceinline__ int parity_of(const int c[4]) { return (c[0] + c[1] + c[2] + c[3]) & 1; }
 
@@ -578,7 +670,8 @@ __device__ __forceinline__ M3 staple(const double2* __restrict__ U, const Geom& 
 {
     const int par = (c[0] + c[1] + c[2] + c[3]) & 1;
     const int s_mu = up(g, s, c[mu], mu);
-    M3 st = m3_zero();
+    M3 st;
+    bool first = true;
 #pragma unroll
     for (int nu = 0; nu < 4; ++nu) {
         if (nu == mu) continue;
@@ -590,14 +683,18 @@ __device__ __forceinline__ M3 staple(const double2* __restrict__ U, const Geom& 
             M3 b = ldu(U, g, s_nu, mu, par ^ 1);
             M3 ab = mul(a, b);
             M3 cc = ldu(U, g, s_mu, nu, par ^ 1);
-            acc(st, mul_nd(ab, cc));
+            if (first)
+                st = mul_nd(ab, cc);
+            else
+                mul_nd_acc(st, ab, cc);
+            first = false;
         }
         {
             M3 a = ldu(U, g, s_nd, nu, par ^ 1);
             M3 b = ldu(U, g, s_nd, mu, par ^ 1);
             M3 ab = mul_dn(a, b);
             M3 cc = ldu(U, g, s_nd_mu, nu, par);
-            acc(st, mul(ab, cc));
+            mul_acc(st, ab, cc);
         }
     }
     return st;

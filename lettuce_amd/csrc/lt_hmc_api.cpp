@@ -297,7 +297,7 @@ int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, doubl
         try {
             for (int k = 0; k <= n_levels; ++k) cv->levels.push_back(alloc_buf(nl));
             for (int k = 0; k < n_levels; ++k) cv->tapes.push_back(alloc_buf(nl));
-            cv->clover = alloc_buf((size_t)108 * cv->g.Vh);
+            cv->clover = alloc_buf((size_t)12 * lt::R_SLOTS * cv->g.Vh);
             cv->lambda = alloc_buf(nl);
         } catch (...) {
             for (auto p : cv->levels) (void)hipFree(p);
@@ -306,6 +306,7 @@ int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, doubl
             delete cv;
             throw;
         }
+        ctx->live += 1;
         *out = cv;
         return LT_OK;
     });
@@ -320,6 +321,7 @@ int lt_cv_destroy(lt_cv cv)
         for (auto p : cv->tapes) (void)hipFree(p);
         (void)hipFree(cv->clover);
         (void)hipFree(cv->lambda);
+        cv->ctx->live -= 1;
         delete cv;
         return LT_OK;
     });

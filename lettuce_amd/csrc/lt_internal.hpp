@@ -63,6 +63,7 @@ struct ProfEntry {
 struct lt_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
+    int live = 0;              // fields, flows, CV workspaces and batches still holding this context
     // device scratch for reduction partials, grown on demand
     double* scratch = nullptr;
     size_t scratch_doubles = 0;

@@ -35,6 +35,8 @@ def gpu_ctx():
     import lettuce_amd as la
     ctx = la.Context(0)
     yield ctx
+    import gc
+    gc.collect()     # objects of finished tests release their handles first
     ctx.close()
 
 

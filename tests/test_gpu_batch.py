@@ -263,3 +263,17 @@ def test_batch_views_reject_destroy_and_swap(la, gpu_ctx):
     with pytest.raises(la.LettuceError):
         b.metad_hmc(2)   # no bias set
     b.close()
+
+
+def test_context_outlives_its_handles(la):
+    ctx = la.Context(0)
+    U = la.GaugeField(ctx, (4, 4, 4, 4))
+    b = la.ReplicaBatch(ctx, [(4, 4, 4, 4)], 1, 0.1)
+    view = b.field(0)
+    with pytest.raises(la.LettuceError):
+        la._lib.call("lt_ctx_destroy", ctx.handle)
+    b.close()
+    with pytest.raises(la.LettuceError):
+        view.SetToIdentity()        # the batch is gone
+    U.close()
+    ctx.close()
