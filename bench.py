@@ -6,7 +6,10 @@ beta = 6.0, thermalised from a Haar-random start (seed 0) with heatbath +
 heatbath sweep + 4 SU(2)-subgroup overrelaxation sweeps (2-colour checkerboard),
 i.e. 5 * 4V link updates.  value = link updates per second over the K timed steps.
 The same run then times RK3 Wilson-flow steps (eps = 0.01) and the fused
-observables pass, reported as extra fields.
+observables pass, reported as extra fields, then (unless skipped) the
+parallel-tempering MetaD workload (16 replicas of 16^4, batched in one graph vs
+one trajectory after another) and the 48^4 workload (HB+OR, RK3 flow to t = 8,
+one MetaD trajectory with 8 stout levels and checkpointing, peak memory).
 
 Roofline: the dominant kernel is the overrelaxation colour kernel (4/5 of the
 sweeps).  Algorithmic bytes per link update = 1296 B (read the 6 staple links of
@@ -15,7 +18,7 @@ direction, read + write the own half: 36 * V * 144 B per sweep over 4V link
 updates); one launch updates V/2 links.  Launch durations come from HIP events
 recorded around every launch on the library's stream.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size L] [--therm N]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size L] [--therm N] [--no-pt] [--no-48]
 """
 import argparse
 import json
@@ -66,6 +69,133 @@ def cpu_baseline(L, beta, budget_s):
             "sample": f"{n_upd} x (1 HB + 4 OR) sweeps of {L}^4 by the OpenMP C oracle, {t_tot:.1f} s"}
 
 
+def thermalised(la, ctx, shape, seed, n_update, beta=6.0):
+    """Haar(seed) followed by n_update HB + 4 OR updates; returns the AoS host array."""
+    U = la.GaugeField(ctx, shape)
+    U.SetToHaar(seed)
+    U.set_rng(1000 + seed, 0)
+    hb = la.HeatbathKernel(U, beta)
+    orx = la.OverrelaxationSubgroupKernel(U)
+    for _ in range(n_update):
+        la.Iterator.Checkerboard(hb, 1)
+        la.Iterator.Checkerboard(orx, 4)
+    data = U.download()
+    U.close()
+    return data
+
+
+def c3_bias(la):
+    """The 16^4 MetaD bias: 800 bins on [-8, 8], height 0.05, width 0.05, plain deposits."""
+    return la.MetaBiasPotential(-8.0, 8.0, 800, 0.05, 0.05, 10.0, 1000.0, well_tempered=False)
+
+
+def pt_metad(la, ctx, L=16, n_rep=16, n_step=20, steps=2):
+    """Parallel-tempering MetaD: n_rep replicas of L^4 (seeds 0..n_rep-1, bias strength r/(n_rep-1),
+    OMF2 tau = 1 with n_step steps, CV = clover Q after 4 stout levels rho = 0.12, path-sample
+    deposits, alternating neighbour exchanges), one trajectory per replica per step, the step
+    replayed as one captured graph; against the same trajectories one replica after another on
+    the single-lattice path (host-synchronised MetaD-HMC, the reference's call structure)."""
+    shape = (L,) * 4
+    starts = [thermalised(la, ctx, shape, r, 20) for r in range(n_rep)]
+    b = la.ReplicaBatch(ctx, [shape] * n_rep, 4, 0.12)
+    for r in range(n_rep):
+        b.field(r).upload(starts[r])
+        b.set_rng(r, r, 0)
+    bias = c3_bias(la)
+    b.set_bias(bias)
+    b.use_graph(True)
+    b.metad_hmc(n_step, tau=1.0, update_bias=True, swap="alternate")
+    b.results()
+    t0 = time.perf_counter()
+    b.metad_hmc(n_step, tau=1.0, update_bias=True, swap="alternate", n_traj=steps)
+    res = b.results()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    Uc, P, F = la.GaugeField(ctx, shape), la.MatrixField(ctx, shape), la.MatrixField(ctx, shape)
+    cv = la.CollectiveVariable(ctx, shape, 4, 0.12)
+    sbias = c3_bias(la)
+    fields, kernels = [], []
+    for r in range(n_rep):
+        U = la.GaugeField(ctx, shape)
+        U.upload(starts[r])
+        U.set_rng(r, 0)
+        fields.append(U)
+        kernels.append(la.HMCMetaDKernel(U, Uc, P, F, sbias, cv, "omf2", 6.0, 1.0, update_bias=True))
+    kernels[0](n_step)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in kernels:
+        k(n_step)
+    ctx.synchronize()
+    t_serial = time.perf_counter() - t0
+    cv.close()
+    for f in fields + [Uc, P, F]:
+        f.close()
+    batched = n_rep * steps / t_batch
+    serial = n_rep / t_serial
+    return {"config": f"{n_rep} x {L}^4, OMF2 tau=1 n_step={n_step}, 4 stout levels, bias x r/{n_rep - 1}",
+            "batched_traj_per_s": batched, "serial_traj_per_s": serial, "ratio": batched / serial,
+            "accept_rate_last_step": float(res["accepted"].mean())}
+
+
+def lattice48(la, ctx, n_update=200, flow_steps=800, interval=3):
+    """48^4 at beta = 6.2: HB + 4 OR updates from Haar(0), RK3 flow to t = 8 with Q, and one
+    MetaD-HMC trajectory (OMF2 tau = 1, 20 steps) with 8 stout levels and per-level checkpointing;
+    peak device memory of the MetaD workspace with and without checkpointing."""
+    L = 48
+    shape = (L,) * 4
+    V = L ** 4
+    U = la.GaugeField(ctx, shape)
+    U.SetToHaar(0)
+    U.set_rng(0, 0)
+    hb = la.HeatbathKernel(U, 6.2)
+    orx = la.OverrelaxationSubgroupKernel(U)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_update):
+        la.Iterator.Checkerboard(hb, 1)
+        la.Iterator.Checkerboard(orx, 4)
+    ctx.synchronize()
+    t_upd = time.perf_counter() - t0
+    W = la.GaugeField(ctx, shape)
+    fl = la.GradientFlowKernel(U, W, "rk3", 0.01)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    fl(flow_steps)
+    q = la.TopChargeClover(W)
+    t_flow = time.perf_counter() - t0
+    fl.close()
+    W.close()
+    fields = [la.GaugeField(ctx, shape), la.MatrixField(ctx, shape), la.MatrixField(ctx, shape)]
+    free0, total = ctx.mem_info()
+    full = la.CollectiveVariable(ctx, shape, 8, 0.12)
+    full_bytes = full.bytes()
+    full.close()
+    cv = la.CollectiveVariable(ctx, shape, 8, 0.12, checkpoint=interval)
+    ck_bytes = cv.bytes()
+    k = la.HMCMetaDKernel(U, fields[0], fields[1], fields[2], c3_bias(la), cv, "omf2", 6.2, 1.0, update_bias=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    k(20)
+    ctx.synchronize()
+    t_md = time.perf_counter() - t0
+    free1, _ = ctx.mem_info()
+    cv.close()
+    for f in fields:
+        f.close()
+    field_bytes = 72 * (V // 2) * 16
+    U.close()
+    return {"lattice": list(shape), "beta": 6.2,
+            "hb_or_link_updates_per_s": 5 * 4 * V * n_update / t_upd, "hb_or_updates": n_update,
+            "flow_rk3_ms_per_step": 1000.0 * t_flow / flow_steps, "flow_steps": flow_steps, "Q_t8": q,
+            "metad_traj_s": t_md, "metad_levels": 8, "checkpoint_interval": interval,
+            "metad_workspace_GB_full": (4 * field_bytes + full_bytes) / 1e9,
+            "metad_workspace_GB_checkpointed": (4 * field_bytes + ck_bytes) / 1e9,
+            "device_used_GB_before_workspace": (total - free0) / 1e9,
+            "device_used_GB_during_trajectory": (total - free1) / 1e9,
+            "reference_model_GB": (10 + 4.33 * 8) * 4 * V * 144 / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +207,8 @@ def main():
     ap.add_argument("--flow-steps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pt", action="store_true", help="skip the 16 x 16^4 PT-MetaD measurement")
+    ap.add_argument("--no-48", action="store_true", help="skip the 48^4 measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -161,6 +293,7 @@ def main():
         la.Observables(W)
     obs_ms = 100.0 * (time.perf_counter() - t0)
     fl.close()
+    W.close()
 
     result = {
         "metric": "FP64 link updates/s (HB + 4 OR sweeps, 32^4)",
@@ -195,6 +328,13 @@ def main():
         "plaquette": plaq,
         "thermalisation_s": t_therm,
     }
+    U.close()
+    if not args.no_pt:
+        print("# PT-MetaD 16 x 16^4", file=sys.stderr, flush=True)
+        result["pt_metad"] = pt_metad(la, ctx)
+    if not args.no_48:
+        print("# 48^4", file=sys.stderr, flush=True)
+        result["lattice_48"] = lattice48(la, ctx)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(L, args.beta, args.cpu_seconds)
     else:

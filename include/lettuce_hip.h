@@ -62,6 +62,8 @@ int lt_ctx_stream(lt_ctx ctx, void **stream);
  * stream; query returns launches and total milliseconds since enable/reset */
 int lt_ctx_profile(lt_ctx ctx, int enable);
 int lt_ctx_profile_query(lt_ctx ctx, const char *kernel, int *launches, double *total_ms);
+/* free and total device memory of the context's GPU (hipMemGetInfo), for peak-memory reports */
+int lt_ctx_mem_info(lt_ctx ctx, size_t *free_bytes, size_t *total_bytes);
 
 /* ---- gauge fields ---------------------------------------------------------
  * GaugeField4D<Nt,Nx,Ny,Nz,Matrix_SU3> (lattice.hpp:87-394) with runtime
@@ -160,6 +162,16 @@ int lt_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, double beta, int 
  * their exponent tapes, the clover planes and the recursion's Lambda buffer. */
 typedef struct lt_cv_s *lt_cv;
 int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, double rho, lt_cv *out);
+/* As lt_cv_create with per-level checkpointing (no reference counterpart; the
+ * reference stores all n+1 levels and n ExpConstants fields, hmc_metadynamics.hpp:
+ * 251-260): only levels interval, 2*interval, ... and the top level are kept, the
+ * backward pass recomputes each segment from its checkpoint (n extra stout passes),
+ * and level 0 is the forward pass's input field, which must stay unchanged until
+ * lt_cv_backward.  interval = 0 is full storage; results are identical either way. */
+int lt_cv_create_checkpointed(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, double rho,
+                              int interval, lt_cv *out);
+/* device memory held by the workspace */
+int lt_cv_bytes(lt_cv cv, size_t *bytes);
 int lt_cv_destroy(lt_cv cv);
 /* StoutSmearing4DWithConstants x n, CalculateClover<1>, TopChargeClover(FullTensor)
  * (stout_smearing.hpp:92-119, clover.hpp:94-152, topological_charge.hpp:202-231) */
@@ -233,6 +245,17 @@ int lt_batch_metad_hmc(lt_batch b, double beta, int integrator, double tau, int 
                        int update_bias, int swap_mode, int n_traj);
 /* last step, per replica (NULL entries skipped): dH, accepted, CV after the step, label */
 int lt_batch_results(lt_batch b, double *dH, int *accepted, double *cv, int *labels);
+/* launch-shape autotuning (no reference counterpart: the reference's OpenMP loops
+ * have no launch shape).  Times the candidate thread-block tiles of the link
+ * kernels on scratch copies and keeps the fastest; results are bit-identical for
+ * every tile.  cache_path (may be NULL) is a JSON file of tuned tiles keyed by GPU
+ * architecture, extents and kernel family; a cached entry is applied without timing. */
+#define LT_TUNE_SWEEP 1   /* heatbath / overrelaxation colour kernels */
+#define LT_TUNE_FLOW  2   /* stout / Wilson-flow stage kernels */
+int lt_autotune_field(lt_field f, int family, const char *cache_path, int tile_out[4], double *best_ms);
+/* per distinct replica extent: the tile of the batched MetaD force kernels */
+int lt_batch_autotune(lt_batch b, const char *cache_path);
+
 /* last step's CV path samples, n_rep rows of (samples per trajectory) */
 int lt_batch_samples(lt_batch b, double *samples, int n);
 /* cumulative exchange statistics: stats[2k] proposed, stats[2k+1] accepted for labels (k, k+1) */

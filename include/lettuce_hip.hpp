@@ -128,4 +128,186 @@ class GradientFlowKernel {
     lt_flow h_ = nullptr;
 };
 
+// A field of general 3x3 matrices (HMC momenta, forces)
+class MatrixField {
+  public:
+    MatrixField(Context& ctx, int nt, int nx, int ny, int nz)
+    {
+        LETTUCE_HIP_CALL(lt_matrix_field_create(ctx.handle(), nt, nx, ny, nz, &h_));
+    }
+    ~MatrixField() { lt_field_destroy(h_); }
+    MatrixField(const MatrixField&) = delete;
+    MatrixField& operator=(const MatrixField&) = delete;
+    lt_field handle() const { return h_; }
+
+  private:
+    lt_field h_ = nullptr;
+};
+
+// HMCKernel(U, U_copy, P, Integrator, Action, prng, ..., trajectory_length)
+// (updates/hmc_gauge.hpp:381-600); momenta from the field's Philox stream
+class HMCKernel {
+  public:
+    HMCKernel(GaugeField& U, GaugeField& U_copy, MatrixField& P, int integrator, double beta, double tau)
+        : U_(U), Uc_(U_copy), P_(P), integrator_(integrator), beta_(beta), tau_(tau)
+    {
+    }
+    bool operator()(int n_step, bool metropolis_step = true)
+    {
+        int acc = 0;
+        LETTUCE_HIP_CALL(lt_hmc_trajectory(U_.handle(), P_.handle(), Uc_.handle(), beta_, integrator_, tau_, n_step,
+                                           metropolis_step ? 1 : 0, &delta_H, &acc));
+        return acc != 0;
+    }
+    double delta_H = 0.0;
+
+  private:
+    GaugeField& U_;
+    GaugeField& Uc_;
+    MatrixField& P_;
+    int integrator_;
+    double beta_, tau_;
+};
+
+// MetaBiasPotential (actions/bias_potential/metadynamics.hpp:29-507); throws on bad ranges like the reference
+class MetaBiasPotential {
+  public:
+    MetaBiasPotential(double cv_min, double cv_max, int bins, double weight, double width, double wt_param,
+                      double threshold, bool well_tempered)
+    {
+        LETTUCE_HIP_CALL(lt_bias_create(cv_min, cv_max, bins, weight, width, well_tempered ? 1 : 0, wt_param,
+                                        threshold, &h_));
+    }
+    ~MetaBiasPotential() { lt_bias_destroy(h_); }
+    MetaBiasPotential(const MetaBiasPotential&) = delete;
+    void UpdatePotential(double cv) { LETTUCE_HIP_CALL(lt_bias_deposit(h_, cv, 0)); }
+    void UpdatePotentialSymmetric(double cv) { LETTUCE_HIP_CALL(lt_bias_deposit(h_, cv, 1)); }
+    double ReturnPotential(double cv) const
+    {
+        double v = 0.0;
+        LETTUCE_HIP_CALL(lt_bias_potential(h_, cv, &v));
+        return v;
+    }
+    double ReturnDerivative(double cv) const
+    {
+        double v = 0.0;
+        LETTUCE_HIP_CALL(lt_bias_derivative(h_, cv, &v));
+        return v;
+    }
+    lt_bias handle() const { return h_; }
+
+  private:
+    lt_bias h_ = nullptr;
+};
+
+// The CV workspace of HMCMetaDData (updates/hmc_metadynamics.hpp:251-260):
+// n stout levels (checkpoint interval 0 = all stored) and the adjoint buffers
+class CollectiveVariable {
+  public:
+    CollectiveVariable(Context& ctx, int nt, int nx, int ny, int nz, int n_levels, double rho, int checkpoint = 0)
+    {
+        LETTUCE_HIP_CALL(lt_cv_create_checkpointed(ctx.handle(), nt, nx, ny, nz, n_levels, rho, checkpoint, &h_));
+    }
+    ~CollectiveVariable() { lt_cv_destroy(h_); }
+    CollectiveVariable(const CollectiveVariable&) = delete;
+    double Forward(const GaugeField& U)
+    {
+        double q = 0.0;
+        LETTUCE_HIP_CALL(lt_cv_forward(h_, U.handle(), &q));
+        return q;
+    }
+    void Backward(double dvdq, MatrixField& F) { LETTUCE_HIP_CALL(lt_cv_backward(h_, dvdq, F.handle())); }
+    lt_cv handle() const { return h_; }
+
+  private:
+    lt_cv h_ = nullptr;
+};
+
+// HMCMetaDKernel(U, U_copy, P, bias, data, Integrator, Action, ..., tau, rho)
+// (updates/hmc_metadynamics.hpp:264-675) with the hand-written adjoint force
+class HMCMetaDKernel {
+  public:
+    HMCMetaDKernel(GaugeField& U, GaugeField& U_copy, MatrixField& P, MatrixField& F, MetaBiasPotential& bias,
+                   CollectiveVariable& cv, int integrator, double beta, double tau, bool update_bias = true)
+        : U_(U), Uc_(U_copy), P_(P), F_(F), bias_(bias), cv_(cv), integrator_(integrator), beta_(beta), tau_(tau),
+          update_bias_(update_bias)
+    {
+    }
+    bool operator()(int n_step, bool metropolis_step = true)
+    {
+        int acc = 0;
+        LETTUCE_HIP_CALL(lt_metad_hmc_trajectory(U_.handle(), P_.handle(), Uc_.handle(), F_.handle(), cv_.handle(),
+                                                 bias_.handle(), beta_, integrator_, tau_, n_step,
+                                                 metropolis_step ? 1 : 0, update_bias_ ? 1 : 0, &delta_H, &acc,
+                                                 &cv_value));
+        return acc != 0;
+    }
+    double delta_H = 0.0, cv_value = 0.0;
+
+  private:
+    GaugeField& U_;
+    GaugeField& Uc_;
+    MatrixField& P_;
+    MatrixField& F_;
+    MetaBiasPotential& bias_;
+    CollectiveVariable& cv_;
+    int integrator_;
+    double beta_, tau_;
+    bool update_bias_;
+};
+
+// Replica batch: the reference's two-stream MetadynamicsTemperingKernel
+// (updates/tempering.hpp:24-73) generalised to a ladder of bias strengths, all
+// replicas advanced by the same launches (optionally one captured graph per step)
+class ReplicaBatch {
+  public:
+    ReplicaBatch(Context& ctx, const std::vector<std::array<int, 4>>& shapes, int n_levels, double rho)
+    {
+        std::vector<int> flat;
+        for (const auto& s : shapes) flat.insert(flat.end(), s.begin(), s.end());
+        LETTUCE_HIP_CALL(lt_batch_create(ctx.handle(), (int)shapes.size(), flat.data(), n_levels, rho, &h_));
+        n_ = (int)shapes.size();
+    }
+    ~ReplicaBatch() { lt_batch_destroy(h_); }
+    ReplicaBatch(const ReplicaBatch&) = delete;
+    // the replica's links in the reference AoS layout
+    void Upload(int r, const double* aos) { LETTUCE_HIP_CALL(lt_field_upload(field(r), aos)); }
+    void Download(int r, double* aos) const { LETTUCE_HIP_CALL(lt_field_download(field(r), aos)); }
+    void SeedPRNG(int r, std::uint64_t seed, std::uint64_t counter = 0)
+    {
+        LETTUCE_HIP_CALL(lt_batch_set_rng(h_, r, seed, counter));
+    }
+    void SetLadder(const std::vector<double>& strengths, std::uint64_t pt_seed = 0)
+    {
+        LETTUCE_HIP_CALL(lt_batch_set_ladder(h_, strengths.data(), nullptr, pt_seed, 0));
+    }
+    void SetBias(const MetaBiasPotential& b) { LETTUCE_HIP_CALL(lt_batch_set_bias(h_, b.handle())); }
+    void GetBias(MetaBiasPotential& b) { LETTUCE_HIP_CALL(lt_batch_get_bias(h_, b.handle())); }
+    void UseGraph(bool on) { LETTUCE_HIP_CALL(lt_batch_use_graph(h_, on ? 1 : 0)); }
+    void Autotune(const char* cache = nullptr) { LETTUCE_HIP_CALL(lt_batch_autotune(h_, cache)); }
+    // n_traj PT-MetaD steps; swap_mode LT_SWAP_*
+    void Step(int n_step, double beta, int integrator, double tau, bool update_bias, int swap_mode, int n_traj = 1)
+    {
+        LETTUCE_HIP_CALL(lt_batch_metad_hmc(h_, beta, integrator, tau, n_step, 1, update_bias ? 1 : 0, swap_mode,
+                                            n_traj));
+    }
+    std::vector<int> Labels() const
+    {
+        std::vector<int> l(n_);
+        LETTUCE_HIP_CALL(lt_batch_results(h_, nullptr, nullptr, nullptr, l.data()));
+        return l;
+    }
+    int Size() const { return n_; }
+
+  private:
+    lt_field field(int r) const
+    {
+        lt_field f = nullptr;
+        LETTUCE_HIP_CALL(lt_batch_field(h_, r, &f));
+        return f;
+    }
+    lt_batch h_ = nullptr;
+    int n_ = 0;
+};
+
 }  // namespace lettuce_hip

@@ -66,6 +66,13 @@ class Context:
     def profile(self, enable=True):
         call("lt_ctx_profile", self._h, 1 if enable else 0)
 
+    def mem_info(self):
+        """(free, total) device memory in bytes."""
+        f = ctypes.c_size_t()
+        t = ctypes.c_size_t()
+        call("lt_ctx_mem_info", self._h, ctypes.byref(f), ctypes.byref(t))
+        return f.value, t.value
+
     def kernel_time(self, name):
         n = ctypes.c_int()
         ms = ctypes.c_double()
@@ -143,6 +150,16 @@ class GaugeField:
         t = (ctypes.c_int * 4)()
         call("lt_field_get_tile", self._h, t)
         return tuple(t)
+
+    def autotune(self, family="sweep", cache=None):
+        """Time the candidate tiles of the sweep or flow kernels on scratch copies of this
+        field, apply the fastest; cache: optional JSON file of tuned tiles.
+        Returns (tile, ms per launch)."""
+        t = (ctypes.c_int * 4)()
+        ms = ctypes.c_double()
+        code = {"sweep": 1, "flow": 2}[family]
+        call("lt_autotune_field", self._h, code, cache.encode() if cache else None, t, ctypes.byref(ms))
+        return tuple(t), ms.value
 
     def get_rng(self):
         s = ctypes.c_uint64()
@@ -365,9 +382,11 @@ class HMCKernel:
 class CollectiveVariable:
     """Workspace of the smeared clover charge and its adjoint (HMCMetaDData, hmc_metadynamics.hpp:251-260)."""
 
-    def __init__(self, ctx, shape, n_levels=4, rho=0.12):
+    def __init__(self, ctx, shape, n_levels=4, rho=0.12, checkpoint=0):
+        """checkpoint = k > 0 keeps only every k-th stout level and recomputes the rest
+        in the backward pass (identical results, less memory)."""
         h = ctypes.c_void_p()
-        call("lt_cv_create", ctx.handle, *shape, n_levels, rho, ctypes.byref(h))
+        call("lt_cv_create_checkpointed", ctx.handle, *shape, n_levels, rho, checkpoint, ctypes.byref(h))
         self._h = h
         self.ctx = ctx      # the context must outlive the workspace
         self.shape = tuple(shape)
@@ -383,6 +402,11 @@ class CollectiveVariable:
 
     def backward(self, dvdq, F):
         call("lt_cv_backward", self._h, dvdq, F.handle)
+
+    def bytes(self):
+        n = ctypes.c_size_t()
+        call("lt_cv_bytes", self._h, ctypes.byref(n))
+        return n.value
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -566,6 +590,10 @@ class ReplicaBatch:
 
     def use_graph(self, enable=True):
         call("lt_batch_use_graph", self._h, 1 if enable else 0)
+
+    def autotune(self, cache=None):
+        """Tune the thread-block tile of the batched MetaD kernels per replica extent."""
+        call("lt_batch_autotune", self._h, cache.encode() if cache else None)
 
     def metad_hmc(self, n_step, beta=6.0, integrator="omf2", tau=1.0, metropolis=True, update_bias=False,
                   swap="none", n_traj=1):

@@ -61,6 +61,9 @@ SIGNATURES = {
     "lt_hmc_trajectory": [c_field, c_field, c_field, ctypes.c_double, I, ctypes.c_double, I, I, D,
                           ctypes.POINTER(I)],
     "lt_cv_create": [c_ctx, I, I, I, I, I, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)],
+    "lt_cv_create_checkpointed": [c_ctx, I, I, I, I, I, ctypes.c_double, I, ctypes.POINTER(ctypes.c_void_p)],
+    "lt_cv_bytes": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)],
+    "lt_ctx_mem_info": [c_ctx, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
     "lt_cv_destroy": [ctypes.c_void_p],
     "lt_cv_forward": [ctypes.c_void_p, c_field, D],
     "lt_cv_backward": [ctypes.c_void_p, ctypes.c_double, c_field],
@@ -88,6 +91,8 @@ SIGNATURES = {
     "lt_batch_results": [ctypes.c_void_p, D, ctypes.POINTER(I), D, ctypes.POINTER(I)],
     "lt_batch_samples": [ctypes.c_void_p, D, I],
     "lt_batch_swap_stats": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), I],
+    "lt_autotune_field": [c_field, I, ctypes.c_char_p, ctypes.POINTER(I), D],
+    "lt_batch_autotune": [ctypes.c_void_p, ctypes.c_char_p],
 }
 
 

@@ -237,6 +237,19 @@ int lt_ctx_stream(lt_ctx ctx, void** stream)
     return LT_OK;
 }
 
+int lt_ctx_mem_info(lt_ctx ctx, size_t* free_bytes, size_t* total_bytes)
+{
+    if (!ctx) return fail(LT_EINVAL, "null context");
+    return guarded([&] {
+        use_device(ctx);
+        size_t f = 0, t = 0;
+        LT_HIP_CHECK(hipMemGetInfo(&f, &t));
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
+        return LT_OK;
+    });
+}
+
 int lt_ctx_profile(lt_ctx ctx, int enable)
 {
     if (!ctx) return fail(LT_EINVAL, "null context");

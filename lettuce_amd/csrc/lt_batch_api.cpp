@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "lt_batch.hpp"
+#include "lt_tune.hpp"
 
 namespace {
 
@@ -56,6 +57,7 @@ struct lt_batch_s {
     lt::BatchRep* d_reps = nullptr;
     int* d_pre = nullptr;
     double2 *U = nullptr, *Ucopy = nullptr, *P = nullptr, *F = nullptr, *lambda = nullptr, *clover = nullptr;
+    double2* fsum = nullptr;
     std::vector<double2*> levels;          // levels[0] aliases U
     std::vector<double2*> tapes;
     double* part = nullptr;
@@ -101,7 +103,8 @@ struct lt_batch_s {
         levels.clear();
         tapes.clear();
         for (void* p : {(void*)d_reps, (void*)d_pre, (void*)U, (void*)Ucopy, (void*)P, (void*)F, (void*)lambda,
-                        (void*)clover, (void*)part, (void*)state_mem, (void*)st.samples, (void*)bias.grid})
+                        (void*)fsum, (void*)clover, (void*)part, (void*)state_mem, (void*)st.samples,
+                        (void*)bias.grid})
             if (p) (void)hipFree(p);
     }
 };
@@ -162,7 +165,7 @@ void enqueue_step(lt_batch b, const StepKey& p, int n_sub)
             lt::kb_clover_deriv(ctx, bg, b->total_lb, b->levels[b->n_levels], b->clover, b->F, st.dvdq);
             for (int k = b->n_levels; k > 0; --k)
                 lt::kb_stout_recursion(ctx, bg, b->total_lb, b->levels[k - 1], b->levels[k], b->F, b->tapes[k - 1],
-                                       b->lambda, b->rho);
+                                       b->lambda, b->fsum, b->rho);
             lt::kb_force_update(ctx, bg, b->total_lb, b->U, b->P, b->F, e * p.beta / 6.0, e);
         },
         [&](double e) { lt::kb_field_update(ctx, bg, b->total_lb, b->U, b->P, e); });
@@ -244,12 +247,13 @@ int lt_batch_create(lt_ctx ctx, int n_rep, const int* shapes, int n_levels, doub
             b->Ucopy = dev_alloc<double2>(nl);
             b->P = dev_alloc<double2>(nl);
             b->F = dev_alloc<double2>(nl);
-            b->lambda = dev_alloc<double2>(nl);
+            b->lambda = dev_alloc<double2>((size_t)lt::alg_offset(b->total));
+            b->fsum = dev_alloc<double2>(nl);
             b->clover = dev_alloc<double2>((size_t)lt::clover_offset(b->total));
             b->levels.push_back(b->U);
             for (int l = 0; l < n_levels; ++l) {
                 b->levels.push_back(dev_alloc<double2>(nl));
-                b->tapes.push_back(dev_alloc<double2>(nl));
+                b->tapes.push_back(dev_alloc<double2>((size_t)lt::alg_offset(b->total)));
             }
             b->part = dev_alloc<double>((size_t)std::max(b->total_sb, b->total_kb));
 
@@ -436,6 +440,59 @@ int lt_batch_get_bias(lt_batch b, lt_bias bias)
         download(bias->grid.data(), b->bias.grid, sizeof(double) * bias->grid.size());
         download(&bias->cv_current, b->st.bias_cv, sizeof(double));
         download(&bias->exceeded, b->st.exceeded, sizeof(unsigned long long));
+        return LT_OK;
+    });
+}
+
+// For each distinct replica extent, time the batched force kernels that dominate a
+// MetaD step (a stout level with tape and one level of the stout-force recursion)
+// with each candidate tile of the batch's block size applied to every replica of
+// that extent.  The batch's fields are scratch here: U (level 0) is only read, the
+// other buffers are recomputed by every trajectory.
+int lt_batch_autotune(lt_batch b, const char* cache_path)
+{
+    if (!b) return fail(LT_EINVAL, "null batch");
+    return guarded([&] {
+        LT_HIP_CHECK(hipSetDevice(b->ctx->device));
+        LT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+        b->drop_graph();
+        std::vector<int> done(b->n, 0);
+        for (int r = 0; r < b->n; ++r) {
+            if (done[r]) continue;
+            const lt::Geom g = b->reps[r].g;
+            std::vector<int> same;
+            for (int q = r; q < b->n; ++q) {
+                bool eq = true;
+                for (int d = 0; d < 4; ++d) eq = eq && b->reps[q].g.L[d] == g.L[d];
+                if (eq) {
+                    same.push_back(q);
+                    done[q] = 1;
+                }
+            }
+            const std::string key = lt::tune_key(b->ctx->device, g, ("batch" + std::to_string(b->nthreads)).c_str());
+            lt::LinkMap best{};
+            double ms = 0.0;
+            auto apply = [&](const lt::LinkMap& m) {
+                for (int q : same) b->reps[q].m = m;
+                upload(b->d_reps, b->reps.data(), sizeof(lt::BatchRep) * b->n);
+            };
+            if (!lt::cached_tile(cache_path, key, g, best, ms)) {
+                const lt::BatchGrid bg = b->grid();
+                auto run = [&](const lt::LinkMap& m) {
+                    apply(m);
+                    double2* tape = b->n_levels > 0 ? b->tapes[0] : b->lambda;
+                    if (b->n_levels > 0)   // never writes level 0 (the replicas' fields)
+                        lt::kb_stout_tape(b->ctx, bg, b->total_lb, b->levels[0], b->levels[1], tape, b->rho);
+                    lt::kb_stout_recursion(b->ctx, bg, b->total_lb, b->U, b->U, b->F, tape, b->lambda, b->fsum,
+                                           b->rho);
+                };
+                best = lt::tune(b->ctx, lt::tile_candidates(g, b->nthreads), run, ms, 2);
+                lt::store_tile(cache_path, key, best, ms);
+            }
+            apply(best);
+            for (int q : same) b->views[q]->map = best;
+        }
+        LT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
         return LT_OK;
     });
 }

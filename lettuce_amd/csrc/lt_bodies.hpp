@@ -165,7 +165,8 @@ __device__ __forceinline__ void body_stout_tape(const double2* __restrict__ Uin,
     M3 Q;
 #pragma unroll
     for (int k = 0; k < 9; ++k) Q.m[k] = cd{rho * A.m[k].im, -rho * A.m[k].re};
-    st_link(T, g, s, mu, par, Q);
+    Q = her8_round(Q);   // forward and adjoint use exactly the stored exponent
+    st_her8(T, g, s, mu, par, Q);
     M3 W = mul(su3_exp(Q), L);
     gram_schmidt(W);
     stu(Uout, g, s, mu, par, W);
@@ -319,7 +320,7 @@ __device__ __forceinline__ void body_lambda(const double2* __restrict__ U, const
     if (!map_link_b(g, m, b, mu, par, c, s)) return;
     M3 sp = mul_dn(ldu(Uprev, g, s, mu, par), ld(Sig, g, s, mu, par));
     st_link(Sig, g, s, mu, par, sp);
-    M3 Q = ld(T, g, s, mu, par);
+    M3 Q = ld_her8(T, g, s, mu, par);
     M3 Q2 = mul(Q, Q);
     ExpC e;
     exp_constants(Q, Q2, e);
@@ -332,13 +333,15 @@ __device__ __forceinline__ void body_lambda(const double2* __restrict__ U, const
     M3 Gam;
 #pragma unroll
     for (int k = 0; k < 9; ++k) Gam.m[k] = t1 * Q.m[k] + t2 * Q2.m[k] + f1 * G.m[k] + f2 * QG.m[k];
-    st_link(Lam, g, s, mu, par, tl_herm(Gam));
+    st_her8(Lam, g, s, mu, par, tl_herm(Gam));
 }
 
 // Sigma_new = Algebra(U (Sigma' exp(iQ) + i rho st^+ Lambda - i rho sum_nu [six-term staple of Lambda]))
-__device__ __forceinline__ void body_sigma(const double2* __restrict__ U, double2* __restrict__ Sig,
-                                           const double2* __restrict__ T, const double2* __restrict__ Lam,
-                                           const Geom& g, const LinkMap& m, int b, double rho)
+// (StoutForceRecursion, stout_smearing.hpp:177-229) in two launches: the six-term
+// sum fs into a scratch field, then the rest.  Together they need more registers
+// than one thread has without spilling.
+__device__ __forceinline__ void body_sigma_fs(const double2* __restrict__ U, const double2* __restrict__ Lam,
+                                              double2* __restrict__ Fs, const Geom& g, const LinkMap& m, int b)
 {
     int mu, par, c[4], s;
     if (!map_link_b(g, m, b, mu, par, c, s)) return;
@@ -354,19 +357,28 @@ __device__ __forceinline__ void body_sigma(const double2* __restrict__ U, double
         const M3 Unu_mu = ldu(U, g, smu, nu, q);
         const M3 tmp1 = mul_nd(Unu_mu, ldu(U, g, snu, mu, q));
         // rows 1 and 6: tmp1 (U_nu(x)^+ L_nu(x) + L_mu(x+nu) U_nu(x)^+)
-        acc(fs, mul(tmp1, add(mul_dn(Unu_x, ld(Lam, g, s, nu, par)), mul_nd(ld(Lam, g, snu, mu, q), Unu_x))));
+        acc(fs, mul(tmp1, add(mul_dn(Unu_x, ld_her8(Lam, g, s, nu, par)), mul_nd(ld_her8(Lam, g, snu, mu, q), Unu_x))));
         // row 5: - L_nu(x+mu) tmp1 U_nu(x)^+
-        fs = sub(fs, mul_nd(mul(ld(Lam, g, smu, nu, q), tmp1), Unu_x));
+        fs = sub(fs, mul_nd(mul(ld_her8(Lam, g, smu, nu, q), tmp1), Unu_x));
         // rows 2, 4, 3: U_nu(x+mu-nu)^+ (U_mu(x-nu)^+ (L_mu(x-nu) - L_nu(x-nu)) + L_nu(x+mu-nu) U_mu(x-nu)^+) U_nu(x-nu)
         const M3 Umu_nd = ldu(U, g, snd, mu, q);
-        M3 tmp2 = mul_dn(Umu_nd, sub(ld(Lam, g, snd, mu, q), ld(Lam, g, snd, nu, q)));
-        M3 inner = add(tmp2, mul_nd(ld(Lam, g, smu_nd, nu, par), Umu_nd));
+        M3 tmp2 = mul_dn(Umu_nd, sub(ld_her8(Lam, g, snd, mu, q), ld_her8(Lam, g, snd, nu, q)));
+        M3 inner = add(tmp2, mul_nd(ld_her8(Lam, g, smu_nd, nu, par), Umu_nd));
         acc(fs, mul(mul_dn(ldu(U, g, smu_nd, nu, par), inner), ldu(U, g, snd, nu, q)));
     }
-    M3 st = staple(U, g, s, c, mu);
-    M3 Q = ld(T, g, s, mu, par);
-    M3 inner = mul(ld(Sig, g, s, mu, par), su3_exp(Q));
-    M3 cl = mul(dag(st), ld(Lam, g, s, mu, par));   // st^+ Lambda
+    st_link(Fs, g, s, mu, par, fs);
+}
+
+__device__ __forceinline__ void body_sigma(const double2* __restrict__ U, double2* __restrict__ Sig,
+                                           const double2* __restrict__ T, const double2* __restrict__ Lam,
+                                           const double2* __restrict__ Fs, const Geom& g, const LinkMap& m, int b,
+                                           double rho)
+{
+    int mu, par, c[4], s;
+    if (!map_link_b(g, m, b, mu, par, c, s)) return;
+    M3 cl = mul_dn(staple(U, g, s, c, mu), ld_her8(Lam, g, s, mu, par));   // st^+ Lambda
+    const M3 fs = ld(Fs, g, s, mu, par);
+    M3 inner = mul(ld(Sig, g, s, mu, par), su3_exp(ld_her8(T, g, s, mu, par)));
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         // + i rho (st^+ Lambda) - i rho fs

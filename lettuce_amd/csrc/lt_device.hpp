@@ -561,6 +561,52 @@ __device__ __forceinline__ void st_alg8(double2* __restrict__ Z, const Geom& g, 
     p[(size_t)3 * g.Vh] = make_double2(z.a12.re, z.a12.im);
 }
 
+// Traceless hermitian fields (the stout exponents Q of the CV tape, the stout-force
+// Lambda) in 8 reals, slots as above:
+//   c: 0 = (Re q00, Re q11), 1 = q01, 2 = q02, 3 = q12;
+// Im q_ii = 0, q_22 = -(q_00 + q_11) and q_ji = conj(q_ij) are implied.
+__device__ __forceinline__ M3 ld_her8(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
+{
+    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    const double2 v0 = p[0], v1 = p[(size_t)g.Vh], v2 = p[(size_t)2 * g.Vh], v3 = p[(size_t)3 * g.Vh];
+    M3 r;
+    r.m[0] = cd{v0.x, 0.0};
+    r.m[4] = cd{v0.y, 0.0};
+    r.m[8] = cd{-(v0.x + v0.y), 0.0};
+    r.m[1] = cd{v1.x, v1.y};
+    r.m[2] = cd{v2.x, v2.y};
+    r.m[5] = cd{v3.x, v3.y};
+    r.m[3] = cd{v1.x, -v1.y};
+    r.m[6] = cd{v2.x, -v2.y};
+    r.m[7] = cd{v3.x, -v3.y};
+    return r;
+}
+__device__ __forceinline__ void st_her8(double2* __restrict__ Z, const Geom& g, int s, int mu, int par, const M3& q)
+{
+    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    p[0] = make_double2(q.m[0].re, q.m[4].re);
+    p[(size_t)g.Vh] = make_double2(q.m[1].re, q.m[1].im);
+    p[(size_t)2 * g.Vh] = make_double2(q.m[2].re, q.m[2].im);
+    p[(size_t)3 * g.Vh] = make_double2(q.m[5].re, q.m[5].im);
+}
+// the matrix that st_her8 followed by ld_her8 reproduces
+__device__ __forceinline__ M3 her8_round(const M3& q)
+{
+    M3 r;
+    r.m[0] = cd{q.m[0].re, 0.0};
+    r.m[4] = cd{q.m[4].re, 0.0};
+    r.m[8] = cd{-(q.m[0].re + q.m[4].re), 0.0};
+    r.m[1] = q.m[1];
+    r.m[2] = q.m[2];
+    r.m[5] = q.m[5];
+    r.m[3] = conj(q.m[1]);
+    r.m[6] = conj(q.m[2]);
+    r.m[7] = conj(q.m[5]);
+    return r;
+}
+// complex-double offset of a replica's 8-real field given its link-field offset
+__host__ __device__ constexpr long long alg_offset(long long link_off) { return link_off / 72 * 32; }
+
 // SU(3) links are stored as rows 0 and 1 only: every update ends with the
 // Gram-Schmidt step whose last row is conj(row0 x row1), so row 2 is rebuilt on
 // load with exactly that formula and never travels through memory.

@@ -4,6 +4,7 @@
 // All lattice work runs in HIP kernels; only scalars (H, Q, dV/dQ) touch the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -24,22 +25,47 @@ double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C);
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
                     double scale_factor);
 void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,
-                       double2* Sig, const double2* T, double2* Lam, double rho);
+                       double2* Sig, const double2* T, double2* Lam, double2* Fs, double rho);
 }  // namespace lt
 
 // Collective-variable workspace: stout levels with their exponent tapes, the six
 // clover planes of the most smeared field and the Lambda buffer of the recursion.
+//
+// Full storage keeps all n+1 levels and n tapes.  With a checkpoint interval k
+// only the levels k, 2k, ... (< n) and the top level n are kept; the backward pass
+// recomputes each segment of at most k levels (and its tapes) from its checkpoint
+// into k+1 segment buffers, costing n extra stout passes per backward pass.  In
+// that mode level 0 is the input field itself, which must not change between
+// lt_cv_forward and lt_cv_backward.
 struct lt_cv_s {
     lt_ctx ctx = nullptr;
     lt::Geom g{};
     lt::LinkMap map{};
     int n_levels = 0;
     double rho = 0.0;
-    std::vector<double2*> levels;   // n+1 link buffers (levels[0] = copy of the input)
-    std::vector<double2*> tapes;    // n exponent buffers
+    int interval = 0;               // 0: full storage, else checkpoint interval k
+    std::vector<double2*> levels;   // full: n+1 levels (levels[0] = copy of the input); else the checkpoints
+    std::vector<double2*> tapes;    // full: n exponent buffers; else k segment tapes
+    std::vector<double2*> seg;      // checkpointed: k+1 segment level buffers
+    double2* top = nullptr;         // checkpointed: level n
+    const double2* input = nullptr; // checkpointed: level 0 (the field of the last forward pass)
     double2* clover = nullptr;      // 6 planes per site
     double2* lambda = nullptr;
+    double2* fsum = nullptr;        // six-term Lambda staple sums of the recursion
+    size_t bytes = 0;               // device memory held
     bool forward_done = false;
+
+    void release()
+    {
+        for (auto p : levels) (void)hipFree(p);
+        for (auto p : tapes) (void)hipFree(p);
+        for (auto p : seg) (void)hipFree(p);
+        for (void* p : {(void*)top, (void*)clover, (void*)lambda, (void*)fsum})
+            if (p) (void)hipFree(p);
+        levels.clear();
+        tapes.clear();
+        seg.clear();
+    }
 };
 
 namespace {
@@ -164,12 +190,35 @@ void copy_field(lt_field dst, lt_field src)
 }
 
 // CV forward: levels with tapes, clover planes, Q (FullTensor route, topological_charge.hpp:202-231)
+// checkpointed mode: the buffer that holds level j after the forward pass, or null
+const double2* ckpt_level(lt_cv cv, int j)
+{
+    if (j == 0) return cv->input;
+    if (j == cv->n_levels) return cv->top;
+    if (j % cv->interval == 0) return cv->levels[j / cv->interval - 1];
+    return nullptr;
+}
+
 double cv_forward(lt_cv cv, lt_field U)
 {
-    lt::k_copy(cv->ctx, U->d, cv->levels[0], (size_t)72 * cv->g.Vh);
-    for (int k = 0; k < cv->n_levels; ++k)
-        lt::k_stout_tape(cv->ctx, cv->g, cv->map, cv->levels[k], cv->levels[k + 1], cv->tapes[k], cv->rho);
-    double Q = lt::k_clover_charge(cv->ctx, cv->g, cv->levels[cv->n_levels], cv->clover);
+    const int n = cv->n_levels;
+    if (cv->interval == 0) {
+        lt::k_copy(cv->ctx, U->d, cv->levels[0], (size_t)72 * cv->g.Vh);
+        for (int k = 0; k < n; ++k)
+            lt::k_stout_tape(cv->ctx, cv->g, cv->map, cv->levels[k], cv->levels[k + 1], cv->tapes[k], cv->rho);
+    } else {
+        // keep checkpoints and the top level; other levels ping-pong in the segment buffers
+        cv->input = U->d;
+        const double2* prev = U->d;
+        for (int j = 1; j <= n; ++j) {
+            double2* out = const_cast<double2*>(ckpt_level(cv, j));
+            if (!out) out = cv->seg[j & 1];
+            lt::k_stout_tape(cv->ctx, cv->g, cv->map, prev, out, cv->tapes[0], cv->rho);
+            prev = out;
+        }
+    }
+    const double2* last = cv->interval == 0 ? cv->levels[n] : (n ? cv->top : U->d);
+    double Q = lt::k_clover_charge(cv->ctx, cv->g, last, cv->clover);
     cv->forward_done = true;
     return Q;
 }
@@ -178,10 +227,25 @@ double cv_forward(lt_cv cv, lt_field U)
 void cv_backward(lt_cv cv, double dvdq, double2* F)
 {
     const int n = cv->n_levels;
-    lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[n], cv->clover, F, dvdq);
-    for (int k = n; k > 0; --k)
-        lt::k_stout_recursion(cv->ctx, cv->g, cv->map, cv->levels[k - 1], cv->levels[k], F, cv->tapes[k - 1],
-                              cv->lambda, cv->rho);
+    if (cv->interval == 0) {
+        lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[n], cv->clover, F, dvdq);
+        for (int k = n; k > 0; --k)
+            lt::k_stout_recursion(cv->ctx, cv->g, cv->map, cv->levels[k - 1], cv->levels[k], F, cv->tapes[k - 1],
+                                  cv->lambda, cv->fsum, cv->rho);
+        return;
+    }
+    lt::k_clover_deriv(cv->ctx, cv->g, cv->map, n ? cv->top : cv->input, cv->clover, F, dvdq);
+    const int k = cv->interval;
+    for (int a = ((n - 1) / k) * k; a >= 0 && n > 0; a -= k) {
+        const int b = std::min(a + k, n);
+        // recompute levels a+1..b and tapes a..b-1 of this segment from checkpoint a
+        auto level = [&](int j) -> const double2* { return j == a ? ckpt_level(cv, a) : cv->seg[j - a]; };
+        for (int j = a + 1; j <= b; ++j)
+            lt::k_stout_tape(cv->ctx, cv->g, cv->map, level(j - 1), cv->seg[j - a], cv->tapes[j - a - 1], cv->rho);
+        for (int j = b; j > a; --j)
+            lt::k_stout_recursion(cv->ctx, cv->g, cv->map, level(j - 1), level(j), F, cv->tapes[j - a - 1],
+                                  cv->lambda, cv->fsum, cv->rho);
+    }
 }
 
 // MetaCharge (hmc_metadynamics.hpp:302-310): n stout levels, clover Q
@@ -280,9 +344,11 @@ int lt_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, double beta, int 
     });
 }
 
-int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, double rho, lt_cv* out)
+int lt_cv_create_checkpointed(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, double rho, int interval,
+                              lt_cv* out)
 {
-    if (!ctx || !out || n_levels < 0 || n_levels > 64) return fail(LT_EINVAL, "lt_cv_create: bad arguments");
+    if (!ctx || !out || n_levels < 0 || n_levels > 64 || interval < 0)
+        return fail(LT_EINVAL, "lt_cv_create: bad arguments");
     if (nt < 2 || nx < 2 || ny < 2 || nz < 2 || (nt | nx | ny | nz) & 1)
         return fail(LT_EINVAL, "lt_cv_create: extents must be even and >= 2");
     return guarded([&] {
@@ -293,16 +359,29 @@ int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, doubl
         cv->map = lt::default_linkmap(cv->g);
         cv->n_levels = n_levels;
         cv->rho = rho;
-        const size_t nl = (size_t)72 * cv->g.Vh;
+        cv->interval = interval > 0 ? std::min(interval, std::max(n_levels, 1)) : 0;
+        const size_t nl = (size_t)72 * cv->g.Vh, na = (size_t)32 * cv->g.Vh;
+        auto take = [&](size_t n) {
+            double2* p = alloc_buf(n);
+            cv->bytes += n * sizeof(double2);
+            return p;
+        };
         try {
-            for (int k = 0; k <= n_levels; ++k) cv->levels.push_back(alloc_buf(nl));
-            for (int k = 0; k < n_levels; ++k) cv->tapes.push_back(alloc_buf(nl));
-            cv->clover = alloc_buf((size_t)12 * lt::R_SLOTS * cv->g.Vh);
-            cv->lambda = alloc_buf(nl);
+            if (cv->interval == 0) {
+                for (int k = 0; k <= n_levels; ++k) cv->levels.push_back(take(nl));
+                for (int k = 0; k < n_levels; ++k) cv->tapes.push_back(take(na));
+            } else {
+                const int k = cv->interval;
+                for (int j = k; j < n_levels; j += k) cv->levels.push_back(take(nl));
+                if (n_levels > 0) cv->top = take(nl);
+                for (int j = 0; j <= k; ++j) cv->seg.push_back(take(nl));
+                for (int j = 0; j < k; ++j) cv->tapes.push_back(take(na));
+            }
+            cv->clover = take((size_t)12 * lt::R_SLOTS * cv->g.Vh);
+            cv->lambda = take(na);
+            cv->fsum = take(nl);
         } catch (...) {
-            for (auto p : cv->levels) (void)hipFree(p);
-            for (auto p : cv->tapes) (void)hipFree(p);
-            if (cv->clover) (void)hipFree(cv->clover);
+            cv->release();
             delete cv;
             throw;
         }
@@ -312,15 +391,24 @@ int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, doubl
     });
 }
 
+int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, double rho, lt_cv* out)
+{
+    return lt_cv_create_checkpointed(ctx, nt, nx, ny, nz, n_levels, rho, 0, out);
+}
+
+int lt_cv_bytes(lt_cv cv, size_t* bytes)
+{
+    if (!cv || !bytes) return fail(LT_EINVAL, "null argument");
+    *bytes = cv->bytes;
+    return LT_OK;
+}
+
 int lt_cv_destroy(lt_cv cv)
 {
     if (!cv) return LT_OK;
     return guarded([&] {
         LT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
-        for (auto p : cv->levels) (void)hipFree(p);
-        for (auto p : cv->tapes) (void)hipFree(p);
-        (void)hipFree(cv->clover);
-        (void)hipFree(cv->lambda);
+        cv->release();
         cv->ctx->live -= 1;
         delete cv;
         return LT_OK;

@@ -1,0 +1,222 @@
+// lt_tune.cpp -- launch-shape autotuner for the link kernels.
+//
+// A link kernel's thread block covers a 4D tile of (t, x, y, z/2) sites (LinkMap);
+// every tile gives bit-identical results, only the L2 reuse and occupancy differ.
+// The tuner times the candidate tiles of a lattice on a scratch copy of the data,
+// keeps the fastest, and records it in a small JSON cache keyed by GPU
+// architecture, extents and kernel family, so later runs apply it without timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "lt_tune.hpp"
+
+namespace lt {
+
+std::vector<LinkMap> tile_candidates(const Geom& g, int threads)
+{
+    const int ext[4] = {g.L[0], g.L[1], g.L[2], g.L[3] / 2};
+    std::vector<LinkMap> out;
+    for (int n : {64, 128, 256}) {
+        if (threads > 0 && n != threads) continue;
+        for (int tz = 1; tz <= ext[3] && tz <= n; tz *= 2) {
+            if (ext[3] % tz || (ext[3] % 4 == 0 && tz < 4)) continue;   // keep z rows coalesced
+            for (int ty = 1; ty <= ext[2] && tz * ty <= n; ty *= 2) {
+                if (ext[2] % ty) continue;
+                for (int tx = 1; tx <= ext[1] && tz * ty * tx <= n; tx *= 2) {
+                    if (ext[1] % tx) continue;
+                    const int tt = n / (tz * ty * tx);
+                    LinkMap m{};
+                    if (tt >= 1 && tt * tz * ty * tx == n && make_linkmap(g, tt, tx, ty, tz, m)) out.push_back(m);
+                }
+            }
+        }
+    }
+    if (out.empty()) out.push_back(default_linkmap(g));
+    return out;
+}
+
+std::string tune_key(int device, const Geom& g, const char* family)
+{
+    hipDeviceProp_t p{};
+    std::string arch = "gpu";
+    if (hipGetDeviceProperties(&p, device) == hipSuccess) {
+        arch = p.gcnArchName;
+        arch = arch.substr(0, arch.find(':'));
+    }
+    std::ostringstream s;
+    s << arch << '|' << g.L[0] << 'x' << g.L[1] << 'x' << g.L[2] << 'x' << g.L[3] << '|' << family;
+    return s.str();
+}
+
+// Cache file: one JSON object, "key": [tt, tx, ty, tz, ms] per entry.
+static std::map<std::string, std::array<double, 5>> read_cache(const char* path)
+{
+    std::map<std::string, std::array<double, 5>> m;
+    if (!path || !*path) return m;
+    std::ifstream f(path);
+    if (!f) return m;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string s = ss.str();
+    size_t i = 0;
+    while ((i = s.find('"', i)) != std::string::npos) {
+        const size_t j = s.find('"', i + 1);
+        if (j == std::string::npos) break;
+        const std::string key = s.substr(i + 1, j - i - 1);
+        const size_t a = s.find('[', j), b = s.find(']', j);
+        if (a == std::string::npos || b == std::string::npos || b < a) break;
+        std::array<double, 5> v{};
+        std::istringstream in(s.substr(a + 1, b - a - 1));
+        std::string tok;
+        int n = 0;
+        while (n < 5 && std::getline(in, tok, ',')) v[n++] = std::atof(tok.c_str());
+        if (n == 5) m[key] = v;
+        i = b + 1;
+    }
+    return m;
+}
+
+static void write_cache(const char* path, const std::map<std::string, std::array<double, 5>>& m)
+{
+    if (!path || !*path) return;
+    std::ofstream f(path, std::ios::trunc);
+    if (!f) throw LtError{LT_EINVAL, std::string("cannot write the tuning cache ") + path};
+    f << "{\n";
+    size_t k = 0;
+    for (const auto& e : m) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "[%d, %d, %d, %d, %.6g]", (int)e.second[0], (int)e.second[1],
+                      (int)e.second[2], (int)e.second[3], e.second[4]);
+        f << "  \"" << e.first << "\": " << buf << (++k < m.size() ? ",\n" : "\n");
+    }
+    f << "}\n";
+}
+
+bool cached_tile(const char* path, const std::string& key, const Geom& g, LinkMap& m, double& ms)
+{
+    const auto c = read_cache(path);
+    const auto it = c.find(key);
+    if (it == c.end()) return false;
+    const auto& v = it->second;
+    if (!make_linkmap(g, (int)v[0], (int)v[1], (int)v[2], (int)v[3], m)) return false;
+    ms = v[4];
+    return true;
+}
+
+void store_tile(const char* path, const std::string& key, const LinkMap& m, double ms)
+{
+    if (!path || !*path) return;
+    auto c = read_cache(path);
+    c[key] = {(double)m.T[0], (double)m.T[1], (double)m.T[2], (double)m.T[3], ms};
+    write_cache(path, c);
+}
+
+LinkMap tune(lt_ctx ctx, const std::vector<LinkMap>& cands, const std::function<void(const LinkMap&)>& run,
+             double& best_ms, int reps)
+{
+    hipEvent_t e0, e1;
+    LT_HIP_CHECK(hipEventCreate(&e0));
+    LT_HIP_CHECK(hipEventCreate(&e1));
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    LinkMap best = cands.front();
+    best_ms = 1e30;
+    try {
+        for (const LinkMap& m : cands) {
+            run(m);   // warm-up
+            LT_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < reps; ++r) run(m);
+            LT_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+            LT_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            LT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / reps < best_ms) {
+                best_ms = ms / reps;
+                best = m;
+            }
+        }
+    } catch (...) {
+        ctx->profiling = prof;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    ctx->profiling = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+}  // namespace lt
+
+namespace {
+
+double2* scratch_links(size_t n)
+{
+    double2* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(double2)) != hipSuccess) throw lt::LtError{LT_ENOMEM, "tuner scratch allocation failed"};
+    return p;
+}
+
+}  // namespace
+
+extern "C" int lt_autotune_field(lt_field f, int family, const char* cache_path, int tile_out[4], double* best_ms)
+{
+    if (!f || (family != LT_TUNE_SWEEP && family != LT_TUNE_FLOW))
+        return lt::fail(LT_EINVAL, "lt_autotune_field: bad arguments");
+    return lt::guarded([&] {
+        LT_HIP_CHECK(hipSetDevice(f->ctx->device));
+        const char* fam = family == LT_TUNE_SWEEP ? "sweep" : "flow";
+        const std::string key = lt::tune_key(f->ctx->device, f->g, fam);
+        lt::LinkMap m{};
+        double ms = 0.0;
+        if (!lt::cached_tile(cache_path, key, f->g, m, ms)) {
+            lt_ctx ctx = f->ctx;
+            const size_t nl = (size_t)72 * f->g.Vh;
+            // work on scratch copies: the field itself is not modified
+            double2* a = scratch_links(nl);
+            double2* b = nullptr;
+            double2* z = nullptr;
+            try {
+                lt::k_copy(ctx, f->d, a, nl);
+                lt_field_s tmp = *f;
+                tmp.d = a;
+                tmp.view = true;
+                if (family == LT_TUNE_SWEEP) {
+                    m = lt::tune(ctx, lt::tile_candidates(f->g, 0), [&](const lt::LinkMap& c) {
+                        tmp.map = c;
+                        lt::k_or_sweep(&tmp, 1, LT_CB2);
+                    }, ms, 3);
+                } else {
+                    b = scratch_links(nl);
+                    z = scratch_links((size_t)32 * f->g.Vh);
+                    m = lt::tune(ctx, lt::tile_candidates(f->g, 0), [&](const lt::LinkMap& c) {
+                        lt::k_flow_stage(ctx, f->g, c, a, b, z, 0.0, 0.01, 0.25, 0, 1);
+                    }, ms, 3);
+                }
+            } catch (...) {
+                (void)hipStreamSynchronize(ctx->stream);
+                for (double2* p : {a, b, z})
+                    if (p) (void)hipFree(p);
+                throw;
+            }
+            LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            for (double2* p : {a, b, z})
+                if (p) (void)hipFree(p);
+            lt::store_tile(cache_path, key, m, ms);
+        }
+        f->map = m;
+        if (tile_out)
+            for (int i = 0; i < 4; ++i) tile_out[i] = m.T[i];
+        if (best_ms) *best_ms = ms;
+        return LT_OK;
+    });
+}

@@ -1,0 +1,23 @@
+// lt_tune.hpp -- launch-shape autotuner shared by fields and replica batches.
+#pragma once
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "lt_internal.hpp"
+
+namespace lt {
+
+// tiles of `threads` threads (0: 64, 128 and 256) that divide the extents, z-major
+std::vector<LinkMap> tile_candidates(const Geom& g, int threads);
+// "<arch>|<Nt>x<Nx>x<Ny>x<Nz>|<family>"
+std::string tune_key(int device, const Geom& g, const char* family);
+// cache lookup / update (no-op without a path)
+bool cached_tile(const char* path, const std::string& key, const Geom& g, LinkMap& m, double& ms);
+void store_tile(const char* path, const std::string& key, const LinkMap& m, double ms);
+// times run(m) for every candidate with HIP events on the context stream; returns the fastest
+LinkMap tune(lt_ctx ctx, const std::vector<LinkMap>& cands, const std::function<void(const LinkMap&)>& run,
+             double& best_ms, int reps);
+
+}  // namespace lt

@@ -21,5 +21,20 @@ int main()
     GradientFlowKernel Flow(U, W, LT_FLOW_RK3, 0.01);
     Flow(10);
     std::printf("plaquette %.12f Q %.12f\n", PlaquetteSum(U) / (18.0 * U.Volume()), TopChargeClover(W));
+
+    // MetaD-HMC (SU3_main.cpp:469-546) and two-stream tempering (SU3_main.cpp:549-621)
+    GaugeField U_copy(gpu, 8, 8, 8, 8);
+    MatrixField P(gpu, 8, 8, 8, 8), F(gpu, 8, 8, 8, 8);
+    MetaBiasPotential Bias(-8.0, 8.0, 800, 0.05, 0.08, 10.0, 1000.0, true);
+    CollectiveVariable CV(gpu, 8, 8, 8, 8, 4, 0.12);
+    HMCMetaDKernel HMC_MetaD(U, U_copy, P, F, Bias, CV, LT_INT_OMF2, 6.0, 1.0);
+    const bool accepted = HMC_MetaD(10);
+    ReplicaBatch Temper(gpu, {{8, 8, 8, 8}, {8, 8, 8, 8}}, 4, 0.12);
+    Temper.SetLadder({0.0, 1.0});
+    Temper.SetBias(Bias);
+    Temper.UseGraph(true);
+    Temper.Step(10, 6.0, LT_INT_OMF2, 1.0, true, LT_SWAP_EVEN);
+    std::printf("MetaD accepted %d dH %.6f labels %d %d\n", (int)accepted, HMC_MetaD.delta_H, Temper.Labels()[0],
+                Temper.Labels()[1]);
     return 0;
 }

@@ -254,3 +254,43 @@ def test_metad_trajectory_vs_oracle_small(la, lto, golden, gpu_ctx):
     assert abs(k.cv_value - cv1) <= 1e-10
     assert rel_err(U.download(), Ur) <= 1e-10
     cv.close()
+
+
+@pytest.mark.parametrize("n_levels,interval", [(4, 1), (4, 2), (4, 3), (8, 3), (5, 5), (1, 1)])
+def test_checkpointed_cv_equals_full_storage(la, lto, gpu_ctx, n_levels, interval):
+    """Recomputing stout levels from checkpoints gives the same Q and force bit for bit,
+    with less memory."""
+    flush(f"checkpointed CV n={n_levels} k={interval}")
+    U0 = smooth(lto, SMALL, 12, 1)
+    U = gauge(la, gpu_ctx, SMALL, U0)
+    F_full, F_ck = matrix(la, gpu_ctx, SMALL), matrix(la, gpu_ctx, SMALL)
+    full = la.CollectiveVariable(gpu_ctx, SMALL, n_levels, 0.12)
+    ck = la.CollectiveVariable(gpu_ctx, SMALL, n_levels, 0.12, checkpoint=interval)
+    q_full, q_ck = full.forward(U), ck.forward(U)
+    full.backward(0.7, F_full)
+    ck.backward(0.7, F_ck)
+    assert q_full == q_ck
+    assert np.array_equal(F_full.download(), F_ck.download())
+    if (n_levels, interval) == (8, 3):     # about sqrt(n): 7 level and 3 tape buffers instead of 9 and 8
+        assert ck.bytes() < 0.75 * full.bytes()
+    full.close()
+    ck.close()
+
+
+def test_checkpointed_metad_trajectory_equals_full(la, lto, golden, gpu_ctx):
+    flush("checkpointed MetaD trajectory")
+    U0 = smooth(lto, SMALL, 13, 2)
+    out = []
+    for interval in (0, 3):
+        bias, _ = metad_setup(la, lto, golden)
+        U = gauge(la, gpu_ctx, SMALL, U0)
+        Uc, P, F = gauge(la, gpu_ctx, SMALL), matrix(la, gpu_ctx, SMALL), matrix(la, gpu_ctx, SMALL)
+        cv = la.CollectiveVariable(gpu_ctx, SMALL, 8, 0.12, checkpoint=interval)
+        U.set_rng(21, 0)
+        k = la.HMCMetaDKernel(U, Uc, P, F, bias, cv, "omf2", 6.0, 0.3, update_bias=True)
+        acc = k(3, metropolis_step=True)
+        out.append((k.delta_H, acc, k.cv_value, U.download(), bias.grid()))
+        cv.close()
+    (d0, a0, q0, u0, g0), (d1, a1, q1, u1, g1) = out
+    assert d0 == d1 and a0 == a1 and q0 == q1
+    assert np.array_equal(u0, u1) and np.array_equal(g0, g1)
