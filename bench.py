@@ -89,7 +89,7 @@ def c3_bias(la):
     return la.MetaBiasPotential(-8.0, 8.0, 800, 0.05, 0.05, 10.0, 1000.0, well_tempered=False)
 
 
-def pt_metad(la, ctx, L=16, n_rep=16, n_step=20, steps=2):
+def pt_metad(la, ctx, L=16, n_rep=16, n_step=20, steps=2, graph=True):
     """Parallel-tempering MetaD: n_rep replicas of L^4 (seeds 0..n_rep-1, bias strength r/(n_rep-1),
     OMF2 tau = 1 with n_step steps, CV = clover Q after 4 stout levels rho = 0.12, path-sample
     deposits, alternating neighbour exchanges), one trajectory per replica per step, the step
@@ -103,7 +103,7 @@ def pt_metad(la, ctx, L=16, n_rep=16, n_step=20, steps=2):
         b.set_rng(r, r, 0)
     bias = c3_bias(la)
     b.set_bias(bias)
-    b.use_graph(True)
+    b.use_graph(graph)
     b.metad_hmc(n_step, tau=1.0, update_bias=True, swap="alternate")
     b.results()
     t0 = time.perf_counter()
@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="skip the 16 x 16^4 PT-MetaD measurement")
     ap.add_argument("--no-48", action="store_true", help="skip the 48^4 measurement")
+    ap.add_argument("--eager", action="store_true",
+                    help="PT-MetaD steps as eager launches instead of graph replays (kernel tracing)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -331,7 +333,8 @@ def main():
     U.close()
     if not args.no_pt:
         print("# PT-MetaD 16 x 16^4", file=sys.stderr, flush=True)
-        result["pt_metad"] = pt_metad(la, ctx)
+        result["pt_metad"] = pt_metad(la, ctx, graph=not args.eager)
+        result["pt_metad"]["graph"] = not args.eager
     if not args.no_48:
         print("# 48^4", file=sys.stderr, flush=True)
         result["lattice_48"] = lattice48(la, ctx)

@@ -494,22 +494,45 @@ __device__ __forceinline__ int dn(const Geom& g, int s, int cmu, int mu)
     return (cmu > 0) ? s - g.st[mu] : s + (g.L[mu] - 1) * g.st[mu];
 }
 
+// Element i of a buffer.  With LT_ADDR32 the byte offset is a 32-bit value added to
+// the (wave-uniform) base pointer, which lets loads use the scalar-base +
+// vector-offset form; buffers must then stay below 4 GiB.
+#ifndef LT_ADDR32
+#define LT_ADDR32 0
+#endif
+__device__ __forceinline__ const double2& elem(const double2* __restrict__ base, size_t i)
+{
+#if LT_ADDR32
+    return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) + (uint32_t)((uint32_t)i * 16u));
+#else
+    return base[i];
+#endif
+}
+__device__ __forceinline__ double2& elem(double2* __restrict__ base, size_t i)
+{
+#if LT_ADDR32
+    return *reinterpret_cast<double2*>(reinterpret_cast<char*>(base) + (uint32_t)((uint32_t)i * 16u));
+#else
+    return base[i];
+#endif
+}
+
 __device__ __forceinline__ M3 ld(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
-    const double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
     M3 r;
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-        double2 v = p[(size_t)c * g.Vh];
+        double2 v = elem(U, o + (size_t)c * g.Vh);
         r.m[c] = cd{v.x, v.y};
     }
     return r;
 }
 __device__ __forceinline__ void st_link(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
 {
-    double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) p[(size_t)c * g.Vh] = make_double2(a.m[c].re, a.m[c].im);
+    for (int c = 0; c < 9; ++c) elem(U, o + (size_t)c * g.Vh) = make_double2(a.m[c].re, a.m[c].im);
 }
 // su(3) algebra fields (anti-hermitian, traceless: the Wilson-flow Z) in 8 reals,
 //   Z[((mu*2 + parity)*4 + c)*Vh + cb],  c: 0 = (Im a00, Im a11), 1 = a01, 2 = a02, 3 = a12;
@@ -619,11 +642,11 @@ __device__ __forceinline__ void third_row(M3& a)
 }
 __device__ __forceinline__ M3 ldu(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
-    const double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
     M3 r;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-        double2 v = p[(size_t)c * g.Vh];
+        double2 v = elem(U, o + (size_t)c * g.Vh);
         r.m[c] = cd{v.x, v.y};
     }
     third_row(r);
@@ -631,9 +654,9 @@ __device__ __forceinline__ M3 ldu(const double2* __restrict__ U, const Geom& g, 
 }
 __device__ __forceinline__ void stu(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
 {
-    double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) p[(size_t)c * g.Vh] = make_double2(a.m[c].re, a.m[c].im);
+    for (int c = 0; c < 6; ++c) elem(U, o + (size_t)c * g.Vh) = make_double2(a.m[c].re, a.m[c].im);
 }
 
 // Clover planes R_munu = C_munu - C_munu^+ (mu < nu) are stored anti-hermitian in

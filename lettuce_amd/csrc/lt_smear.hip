@@ -25,7 +25,13 @@ __global__ void LT_LINK_KERNEL stout_kernel(const double2* __restrict__ Uin, dou
     stu(Uout, g, s, mu, par, W);
 }
 
-__global__ void LT_LINK_KERNEL flow_stage_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout,
+// The flow stage is VALU-bound; at three waves per SIMD it fits without spills and
+// runs 4 % faster than at two (measured at 32^4; four waves spill and run 1.8x slower).
+#ifndef LT_FLOW_WAVES
+#define LT_FLOW_WAVES 3
+#endif
+__global__ void __launch_bounds__(LT_LINK_BS) __attribute__((amdgpu_waves_per_eu(LT_FLOW_WAVES)))
+flow_stage_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout,
                                                  double2* __restrict__ Z, Geom g, LinkMap m, double c_old, double c_new,
                                                  double c_exp, int read_z, int write_z)
 {

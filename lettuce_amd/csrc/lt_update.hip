@@ -120,9 +120,13 @@ __device__ __forceinline__ void update_link(double2* __restrict__ U, const Geom&
     stu(U, g, s, mu, par, L);
 }
 
+#ifndef LT_SWEEP_WAVES
+#define LT_SWEEP_WAVES LT_LINK_WAVES
+#endif
 template <bool HEATBATH>
-__global__ void LT_LINK_KERNEL sweep_cb2_kernel(double2* __restrict__ U, Geom g, LinkMap m, int mu, int par,
-                                                 double prefactor, uint64_t seed, uint64_t counter)
+__global__ void __launch_bounds__(LT_LINK_BS) __attribute__((amdgpu_waves_per_eu(LT_SWEEP_WAVES)))
+sweep_cb2_kernel(double2* __restrict__ U, Geom g, LinkMap m, int mu, int par, double prefactor, uint64_t seed,
+                 uint64_t counter)
 {
     int c[4], s;
     if (!map_site(g, m, par, c, s)) return;
