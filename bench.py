@@ -26,6 +26,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -89,34 +91,38 @@ def c3_bias(la):
     return la.MetaBiasPotential(-8.0, 8.0, 800, 0.05, 0.05, 10.0, 1000.0, well_tempered=False)
 
 
-def pt_metad(la, ctx, L=16, n_rep=16, n_step=20, steps=2, graph=True):
-    """Parallel-tempering MetaD: n_rep replicas of L^4 (seeds 0..n_rep-1, bias strength r/(n_rep-1),
-    OMF2 tau = 1 with n_step steps, CV = clover Q after 4 stout levels rho = 0.12, path-sample
-    deposits, alternating neighbour exchanges), one trajectory per replica per step, the step
-    replayed as one captured graph; against the same trajectories one replica after another on
-    the single-lattice path (host-synchronised MetaD-HMC, the reference's call structure)."""
-    shape = (L,) * 4
-    starts = [thermalised(la, ctx, shape, r, 20) for r in range(n_rep)]
-    b = la.ReplicaBatch(ctx, [shape] * n_rep, 4, 0.12)
-    for r in range(n_rep):
+def pt_metad(la, ctx, shapes, n_step=20, steps=2, graph=True):
+    """Parallel-tempering MetaD over replicas of the given extents (seeds 0..n-1, bias strength
+    r/(n-1), OMF2 tau = 1 with n_step steps, CV = clover Q after 4 stout levels rho = 0.12,
+    path-sample deposits, alternating exchanges between equal extents), one trajectory per replica
+    per step, each step replayed as one captured graph; against the same trajectories one replica
+    after another on the single-lattice path (host-synchronised MetaD-HMC, the reference's call
+    structure).  Rates in trajectories/s and site-trajectories/s."""
+    n = len(shapes)
+    sites = sum(int(np.prod(s)) for s in shapes)
+    starts = [thermalised(la, ctx, s, r, 20) for r, s in enumerate(shapes)]
+    b = la.ReplicaBatch(ctx, shapes, 4, 0.12)
+    for r in range(n):
         b.field(r).upload(starts[r])
         b.set_rng(r, r, 0)
-    bias = c3_bias(la)
-    b.set_bias(bias)
+    b.set_bias(c3_bias(la))
     b.use_graph(graph)
     b.metad_hmc(n_step, tau=1.0, update_bias=True, swap="alternate")
     b.results()
     t0 = time.perf_counter()
     b.metad_hmc(n_step, tau=1.0, update_bias=True, swap="alternate", n_traj=steps)
     res = b.results()
-    t_batch = time.perf_counter() - t0
+    t_batch = (time.perf_counter() - t0) / steps
     b.close()
-    Uc, P, F = la.GaugeField(ctx, shape), la.MatrixField(ctx, shape), la.MatrixField(ctx, shape)
-    cv = la.CollectiveVariable(ctx, shape, 4, 0.12)
     sbias = c3_bias(la)
+    work = {}   # per extent: U_copy, P, F, CV workspace
     fields, kernels = [], []
-    for r in range(n_rep):
-        U = la.GaugeField(ctx, shape)
+    for r, s in enumerate(shapes):
+        if s not in work:
+            work[s] = (la.GaugeField(ctx, s), la.MatrixField(ctx, s), la.MatrixField(ctx, s),
+                       la.CollectiveVariable(ctx, s, 4, 0.12))
+        Uc, P, F, cv = work[s]
+        U = la.GaugeField(ctx, s)
         U.upload(starts[r])
         U.set_rng(r, 0)
         fields.append(U)
@@ -128,14 +134,37 @@ def pt_metad(la, ctx, L=16, n_rep=16, n_step=20, steps=2, graph=True):
         k(n_step)
     ctx.synchronize()
     t_serial = time.perf_counter() - t0
-    cv.close()
-    for f in fields + [Uc, P, F]:
+    for Uc, P, F, cv in work.values():
+        cv.close()
+        fields += [Uc, P, F]
+    for f in fields:
         f.close()
-    batched = n_rep * steps / t_batch
-    serial = n_rep / t_serial
-    return {"config": f"{n_rep} x {L}^4, OMF2 tau=1 n_step={n_step}, 4 stout levels, bias x r/{n_rep - 1}",
-            "batched_traj_per_s": batched, "serial_traj_per_s": serial, "ratio": batched / serial,
-            "accept_rate_last_step": float(res["accepted"].mean())}
+    return {"replicas": [list(s) for s in sorted(set(shapes))], "n_replicas": n, "sites": sites,
+            "config": f"OMF2 tau=1 n_step={n_step}, 4 stout levels rho=0.12, bias x r/{n - 1}",
+            "batched_traj_per_s": n / t_batch, "serial_traj_per_s": n / t_serial, "ratio": t_serial / t_batch,
+            "batched_site_traj_per_s": sites / t_batch, "accept_rate_last_step": float(res["accepted"].mean())}
+
+
+def lattice8(la, ctx, n_update=200):
+    """8^4 at beta = 5.7: n_update HB + 4 OR updates from Haar(0), plaquette after each (the
+    correctness configuration: launch-latency bound)."""
+    shape = (8,) * 4
+    U = la.GaugeField(ctx, shape)
+    U.SetToHaar(0)
+    U.set_rng(0, 0)
+    hb = la.HeatbathKernel(U, 5.7)
+    orx = la.OverrelaxationSubgroupKernel(U)
+    plaq = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_update):
+        la.Iterator.Checkerboard(hb, 1)
+        la.Iterator.Checkerboard(orx, 4)
+        plaq.append(la.PlaquetteSum(U) / (18.0 * 8 ** 4))
+    dt = time.perf_counter() - t0
+    U.close()
+    return {"lattice": list(shape), "beta": 5.7, "updates": n_update, "ms_per_update": 1000.0 * dt / n_update,
+            "mean_plaquette_last_100": float(np.mean(plaq[-100:]))}
 
 
 def lattice48(la, ctx, n_update=200, flow_steps=800, interval=3):
@@ -333,8 +362,15 @@ def main():
     U.close()
     if not args.no_pt:
         print("# PT-MetaD 16 x 16^4", file=sys.stderr, flush=True)
-        result["pt_metad"] = pt_metad(la, ctx, graph=not args.eager)
+        g16 = (16, 16, 16, 16)
+        result["pt_metad"] = pt_metad(la, ctx, [g16] * 16, graph=not args.eager)
         result["pt_metad"]["graph"] = not args.eager
+        print("# PT-MetaD mixed 16 x 16^4 + 12^4 + 20^4 + 24^4, and 16 x 8^4", file=sys.stderr, flush=True)
+        mixed = [g16] * 16 + [(12,) * 4, (20,) * 4, (24,) * 4]
+        result["pt_metad_mixed"] = pt_metad(la, ctx, mixed, graph=not args.eager)
+        result["pt_metad_8x4"] = pt_metad(la, ctx, [(8,) * 4] * 16, graph=not args.eager)
+        print("# 8^4 correctness configuration", file=sys.stderr, flush=True)
+        result["lattice_8"] = lattice8(la, ctx)
     if not args.no_48:
         print("# 48^4", file=sys.stderr, flush=True)
         result["lattice_48"] = lattice48(la, ctx)
