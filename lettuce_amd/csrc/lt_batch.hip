@@ -18,22 +18,32 @@ namespace {
 constexpr int RBS = 256;   // site / reduction / copy block size
 constexpr int SBS = 64;    // scalar-stage block size
 
+// the replica owning block b: the largest r with pre[r] <= b (every replica owns at least one block)
 __device__ __forceinline__ int rep_of(const int* __restrict__ pre, int n, int b)
 {
-    int r = 0;
-    while (r + 1 < n && b >= pre[r + 1]) ++r;
-    return r;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }
 
 // ------------------------------------------------------------ link kernels
 
-#define LT_BATCH_LINK_PROLOGUE                                  \
-    const int b_ = xcd_block(blockIdx.x, gridDim.x);            \
-    const int r = rep_of(bg.lb, bg.n, b_);                      \
-    const Geom g = bg.reps[r].g;                                \
-    const LinkMap m = bg.reps[r].m;                             \
-    const long long off = bg.reps[r].off;                       \
-    const int lb = b_ - bg.lb[r];
+// Hardware blocks go round-robin over the 8 XCDs and every replica's block range
+// starts at a multiple of 8, so remapping inside each replica gives every XCD a
+// contiguous eighth of every replica: the L2 locality of a lattice run alone, and
+// equal work per XCD whatever the mix of extents.
+#define LT_BATCH_LINK_PROLOGUE                                                  \
+    const int r = rep_of(bg.lb, bg.n, blockIdx.x);                              \
+    const Geom g = bg.reps[r].g;                                                \
+    const LinkMap m = bg.reps[r].m;                                             \
+    const long long off = bg.reps[r].off;                                       \
+    const int lb = xcd_block(blockIdx.x - bg.lb[r], bg.lb[r + 1] - bg.lb[r]);
 
 __global__ void LT_LINK_KERNEL bk_momentum(double2* __restrict__ P, BatchGrid bg, const uint64_t* __restrict__ counter)
 {
