@@ -347,7 +347,8 @@ __device__ __forceinline__ void exp_constants(const M3& Q, const M3& Q2, ExpC& e
     double c0s = (1.0 / 3.0) * trq3;
     double c0 = fabs(c0s);
     double c1 = 0.5 * (Q2.m[0].re + Q2.m[4].re + Q2.m[8].re);
-    e.c0max = 2.0 * pow(c1 / 3.0, 1.5);
+    const double c1_3 = c1 / 3.0;
+    e.c0max = 2.0 * (c1_3 * sqrt(c1_3));   // 2 (c1/3)^(3/2)
     e.signflip = c0s < 0.0;
     double theta = acos(fmin(c0 / e.c0max, 1.0));
     double st, ct;
@@ -396,10 +397,53 @@ __device__ __forceinline__ M3 exp_from(const ExpC& e, const M3& Q, const M3& Q2)
     return r;
 }
 
+// exp(iQ) = f0 + f1 Q + f2 Q^2 for small Q by the power series reduced with
+// Cayley-Hamilton, Q^3 = c1 Q + c0 (c1 = tr Q^2 / 2, c0 = det Q): with
+// Q^k / k! = a_k + b_k Q + g_k Q^2,
+//   a_{k+1} = g_k c0 / (k+1),  b_{k+1} = (a_k + g_k c1) / (k+1),  g_{k+1} = b_k / (k+1),
+// f_j = sum_k i^k (a_k, b_k, g_k).  For c1 <= EXP_SERIES_C1 (|Q|_F <= 0.2) the 13
+// terms k = 0..12 leave a remainder below 0.2^13 / 13! ~ 1e-19; no transcendental
+// functions are needed (the Wilson-flow exponents are always this small).
+constexpr double EXP_SERIES_C1 = 0.02;
+__device__ __forceinline__ void exp_series_coeffs(double c0, double c1, cd& f0, cd& f1, cd& f2)
+{
+    double a = 1.0, b = 0.0, gg = 0.0;
+    double f0r = 0.0, f0i = 0.0, f1r = 0.0, f1i = 0.0, f2r = 0.0, f2i = 0.0;
+#pragma unroll
+    for (int k = 0; k <= 12; ++k) {
+        switch (k & 3) {   // i^k
+            case 0: f0r += a; f1r += b; f2r += gg; break;
+            case 1: f0i += a; f1i += b; f2i += gg; break;
+            case 2: f0r -= a; f1r -= b; f2r -= gg; break;
+            default: f0i -= a; f1i -= b; f2i -= gg; break;
+        }
+        const double inv = 1.0 / (double)(k + 1);
+        const double an = gg * c0 * inv, bn = fma(gg, c1, a) * inv, gn = b * inv;
+        a = an;
+        b = bn;
+        gg = gn;
+    }
+    f0 = cd{f0r, f0i};
+    f1 = cd{f1r, f1i};
+    f2 = cd{f2r, f2i};
+}
+
 // exp(iQ)
 __device__ __forceinline__ M3 su3_exp(const M3& Q)
 {
     M3 Q2 = mul(Q, Q);
+    const double c1 = 0.5 * (Q2.m[0].re + Q2.m[4].re + Q2.m[8].re);
+    if (c1 <= EXP_SERIES_C1) {
+        cd f0, f1, f2;
+        exp_series_coeffs((1.0 / 3.0) * retr_mul(Q, Q2), c1, f0, f1, f2);
+        M3 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.m[i] = f1 * Q.m[i] + f2 * Q2.m[i];
+        r.m[0] += f0;
+        r.m[4] += f0;
+        r.m[8] += f0;
+        return r;
+    }
     ExpC e;
     exp_constants(Q, Q2, e);
     return exp_from(e, Q, Q2);
