@@ -46,6 +46,11 @@ bool make_linkmap(const Geom& g, int tt, int tx, int ty, int tz, LinkMap& m)
         m.N[i] = ext[i] / T[i];
     }
     m.nthreads = n;
+    // one t-slab per XCD when the t tiles split evenly over the eight XCDs
+    m.tc = 1;
+#if LT_TCHUNK
+    if (m.N[0] % 8 == 0) m.tc = m.N[0] / 8;
+#endif
     return true;
 }
 

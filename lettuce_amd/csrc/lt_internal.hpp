@@ -180,7 +180,8 @@ inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / 
 // grid size of a tiled link launch; refuses maps the kernels were not compiled for
 inline unsigned map_blocks(const LinkMap& m)
 {
-    if (m.nthreads < 1 || m.nthreads > LT_LINK_BS || m.T[0] * m.T[1] * m.T[2] * m.T[3] != m.nthreads)
+    if (m.nthreads < 1 || m.nthreads > LT_LINK_BS || m.T[0] * m.T[1] * m.T[2] * m.T[3] != m.nthreads || m.tc < 1
+        || m.N[0] % m.tc)
         throw LtError{LT_ESTATE, "invalid thread-block tile for the link kernels"};
     return (unsigned)(m.N[0] * m.N[1] * m.N[2] * m.N[3]);
 }

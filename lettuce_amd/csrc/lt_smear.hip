@@ -20,7 +20,7 @@ __global__ void LT_LINK_KERNEL stout_kernel(const double2* __restrict__ Uin, dou
     M3 st = staple(Uin, g, s, c, mu);
     M3 L = ldu(Uin, g, s, mu, par);
     M3 A = algebra(mul_nd(st, L));   // C = 1/2 (A - A^+) - 1/6 tr(A - A^+)
-    M3 W = mul(exp_ah(A, rho), L);   // exp(-i rho C) applied as exp(i * (-i rho C))
+    M3 W = mul(exp_alg8(to_alg8(A), rho), L);   // exp(rho C)
     gram_schmidt(W);
     stu(Uout, g, s, mu, par, W);
 }
@@ -42,7 +42,7 @@ flow_stage_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout,
     const Alg8 F = to_alg8(algebra(mul_nd(st, L)));
     const Alg8 z = read_z ? axpby(c_old, ld_alg8(Z, g, s, mu, par), c_new, F) : scale(c_new, F);
     if (write_z) st_alg8(Z, g, s, mu, par, z);
-    M3 W = mul(exp_ah(from_alg8(z), c_exp), L);
+    M3 W = mul(exp_alg8(z, c_exp), L);
     gram_schmidt(W);
     stu(Uout, g, s, mu, par, W);
 }
