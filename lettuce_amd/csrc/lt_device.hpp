@@ -700,6 +700,36 @@ __device__ __forceinline__ M3 exp_alg8(const Alg8& z, double c)
     return r;
 }
 
+// Streaming (non-temporal) accesses for data touched once per launch, so that it
+// does not displace the L2-resident neighbourhood of the link reads.
+typedef double lt_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_stream(const double2* p)
+{
+    const lt_d2v v = __builtin_nontemporal_load(reinterpret_cast<const lt_d2v*>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(double2* p, double2 v)
+{
+    const lt_d2v w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<lt_d2v*>(p));
+}
+__device__ __forceinline__ Alg8 ld_alg8_stream(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
+{
+    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    const double2 v0 = ld_stream(p), v1 = ld_stream(p + (size_t)g.Vh), v2 = ld_stream(p + (size_t)2 * g.Vh),
+                  v3 = ld_stream(p + (size_t)3 * g.Vh);
+    return Alg8{v0.x, v0.y, cd{v1.x, v1.y}, cd{v2.x, v2.y}, cd{v3.x, v3.y}};
+}
+__device__ __forceinline__ void st_alg8_stream(double2* __restrict__ Z, const Geom& g, int s, int mu, int par,
+                                               const Alg8& z)
+{
+    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    st_stream(p, make_double2(z.d0, z.d1));
+    st_stream(p + (size_t)g.Vh, make_double2(z.a01.re, z.a01.im));
+    st_stream(p + (size_t)2 * g.Vh, make_double2(z.a02.re, z.a02.im));
+    st_stream(p + (size_t)3 * g.Vh, make_double2(z.a12.re, z.a12.im));
+}
+
 // Traceless hermitian fields (the stout exponents Q of the CV tape, the stout-force
 // Lambda) in 8 reals, slots as above:
 //   c: 0 = (Re q00, Re q11), 1 = q01, 2 = q02, 3 = q12;
@@ -811,6 +841,13 @@ __device__ __forceinline__ void stu(double2* __restrict__ U, const Geom& g, int 
     for (int c = 0; c < 6; ++c) elem(U, o + (size_t)c * g.Vh) = make_double2(a.m[c].re, a.m[c].im);
 }
 
+__device__ __forceinline__ void stu_stream(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
+{
+    double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) st_stream(p + (size_t)c * g.Vh, make_double2(a.m[c].re, a.m[c].im));
+}
+
 // Clover planes R_munu = C_munu - C_munu^+ (mu < nu) are stored anti-hermitian in
 // 9 reals, R_SLOTS complex slots per (plane, parity): 12 * R_SLOTS * Vh complex
 // doubles per lattice; a replica at link offset off (72 * Vh units) has its
@@ -905,29 +942,115 @@ __device__ __forceinline__ M3 staple(const double2* __restrict__ U, const Geom& 
     const int par = (c[0] + c[1] + c[2] + c[3]) & 1;
     const int s_mu = up(g, s, c[mu], mu);
     M3 st;
-    bool first = true;
 #pragma unroll
-    for (int nu = 0; nu < 4; ++nu) {
+    for (int nu = 0, k = 0; nu < 4; ++nu) {
         if (nu == mu) continue;
-        const int s_nu = up(g, s, c[nu], nu);
-        const int s_nd = dn(g, s, c[nu], nu);
+        const int cnu = c[nu];
+        const int s_nu = up(g, s, cnu, nu);
+        const int s_nd = dn(g, s, cnu, nu);
         const int s_nd_mu = up(g, s_nd, c[mu], mu);
         {
             const M3 ab = mul_r01(ldu2(U, g, s, nu, par), ldu(U, g, s_nu, mu, par ^ 1));
             const M3 t = mul_nd_r01(ab, ldu(U, g, s_mu, nu, par ^ 1));
-            if (first) {
+            if (k == 0) {
                 st = t;
                 third_row(st);
             } else {
                 acc_su3_r01(st, t);
             }
-            first = false;
         }
         {
             M3 ef = mul_r01(ldu2(U, g, s_nd, mu, par ^ 1), ldu(U, g, s_nd_mu, nu, par));
             third_row(ef);
             mul_dn_acc(st, ldu(U, g, s_nd, nu, par ^ 1), ef);
         }
+        ++k;
+    }
+    return st;
+}
+
+// Rows 0 and 1 of one link as loaded (six complex entries).
+struct Raw6 {
+    double2 v[6];
+};
+__device__ __forceinline__ void ld_raw(const double2* __restrict__ U, const Geom& g, int s, int mu, int par, Raw6& r)
+{
+    const double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) r.v[c] = p[(size_t)c * g.Vh];
+}
+__device__ __forceinline__ M3 rows01(const Raw6& r)
+{
+    M3 a;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.m[c] = cd{r.v[c].x, r.v[c].y};
+    a.m[6] = a.m[7] = a.m[8] = cd{0.0, 0.0};
+    return a;
+}
+__device__ __forceinline__ M3 su3_of(const Raw6& r)
+{
+    M3 a = rows01(r);
+    third_row(a);
+    return a;
+}
+__device__ __forceinline__ int pick4(const int v[4], int i) { return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3]; }
+
+// The staple of staple() as six terms (upper and lower for each nu != mu, nu
+// ascending, the same products and summation order), software-pipelined: the
+// three links of term j+1 are loaded before term j is multiplied out, with
+// scheduling barriers so the compiler keeps that order instead of hoisting all
+// 19 link loads (which spills).  The own link is loaded with the last term.
+__device__ __forceinline__ M3 staple_pipelined(const double2* __restrict__ U, const Geom& g, int s, const int c[4],
+                                               int mu, int par, Raw6& own)
+{
+    const int q = par ^ 1;
+    const int s_mu = up(g, s, c[mu], mu);
+    int s_nu[3], s_nd[3], s_ndmu[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int nu = k + (k >= mu);
+        const int cnu = pick4(c, nu), L = pick4(g.L, nu), st = pick4(g.st, nu);
+        s_nu[k] = (cnu + 1 < L) ? s + st : s - (L - 1) * st;
+        s_nd[k] = (cnu > 0) ? s - st : s + (L - 1) * st;
+        s_ndmu[k] = up(g, s_nd[k], c[mu], mu);
+    }
+    // term j = 2k: a = U_nu(x) [rows 0, 1], b = U_mu(x+nu), cc = U_nu(x+mu):  (a b) cc^+
+    // term j = 2k+1: a = U_mu(x-nu) [rows 0, 1], b = U_nu(x-nu+mu), cc = U_nu(x-nu):  cc^+ (a b)
+    auto load = [&](int j, Raw6& a, Raw6& b, Raw6& cc) {
+        const int k = j >> 1, nu = k + (k >= mu);
+        if ((j & 1) == 0) {
+            ld_raw(U, g, s, nu, par, a);
+            ld_raw(U, g, s_nu[k], mu, q, b);
+            ld_raw(U, g, s_mu, nu, q, cc);
+        } else {
+            ld_raw(U, g, s_nd[k], mu, q, a);
+            ld_raw(U, g, s_ndmu[k], nu, par, b);
+            ld_raw(U, g, s_nd[k], nu, q, cc);
+        }
+    };
+    Raw6 A[2], B[2], C[2];
+    load(0, A[0], B[0], C[0]);
+    M3 st;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        if (j < 5) load(j + 1, A[(j + 1) & 1], B[(j + 1) & 1], C[(j + 1) & 1]);
+        if (j == 4) ld_raw(U, g, s, mu, par, own);
+        __builtin_amdgcn_sched_barrier(0);
+        const int i = j & 1;
+        if ((j & 1) == 0) {
+            const M3 t = mul_nd_r01(mul_r01(rows01(A[i]), su3_of(B[i])), su3_of(C[i]));
+            if (j == 0) {
+                st = t;
+                third_row(st);
+            } else {
+                acc_su3_r01(st, t);
+            }
+        } else {
+            M3 ef = mul_r01(rows01(A[i]), su3_of(B[i]));
+            third_row(ef);
+            mul_dn_acc(st, su3_of(C[i]), ef);
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
     return st;
 }

@@ -25,10 +25,23 @@ __global__ void LT_LINK_KERNEL stout_kernel(const double2* __restrict__ Uin, dou
     stu(Uout, g, s, mu, par, W);
 }
 
-// The flow stage is VALU-bound; at three waves per SIMD it fits without spills and
-// runs 4 % faster than at two (measured at 32^4; four waves spill and run 1.8x slower).
+// The flow stage is VALU- and latency-bound.  Pipelined staple (default): two
+// waves per SIMD (240 VGPRs, no spills); the plain staple() loop fits three waves
+// (measured at 32^4: pipelined at two waves 3 % faster than plain at three; four
+// waves spill and run 1.4x slower).
+#ifndef LT_FLOW_PIPE
+#define LT_FLOW_PIPE 1
+#endif
 #ifndef LT_FLOW_WAVES
+#if LT_FLOW_PIPE
+#define LT_FLOW_WAVES 2
+#else
 #define LT_FLOW_WAVES 3
+#endif
+#endif
+// Z and the output links are touched once per stage: streaming accesses
+#ifndef LT_FLOW_STREAM
+#define LT_FLOW_STREAM 1
 #endif
 __global__ void __launch_bounds__(LT_LINK_BS) __attribute__((amdgpu_waves_per_eu(LT_FLOW_WAVES)))
 flow_stage_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout,
@@ -37,14 +50,28 @@ flow_stage_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout,
 {
     int mu, par, c[4], s;
     if (!map_link_all(g, m, mu, par, c, s)) return;
+#if LT_FLOW_PIPE
+    Raw6 own;
+    const M3 st = staple_pipelined(Uin, g, s, c, mu, par, own);
+    const M3 L = su3_of(own);
+#else
     M3 st = staple(Uin, g, s, c, mu);
     M3 L = ldu(Uin, g, s, mu, par);
+#endif
     const Alg8 F = to_alg8(algebra(mul_nd(st, L)));
+#if LT_FLOW_STREAM
+    const Alg8 z = read_z ? axpby(c_old, ld_alg8_stream(Z, g, s, mu, par), c_new, F) : scale(c_new, F);
+    if (write_z) st_alg8_stream(Z, g, s, mu, par, z);
+    M3 W = mul(exp_alg8(z, c_exp), L);
+    gram_schmidt(W);
+    stu_stream(Uout, g, s, mu, par, W);
+#else
     const Alg8 z = read_z ? axpby(c_old, ld_alg8(Z, g, s, mu, par), c_new, F) : scale(c_new, F);
     if (write_z) st_alg8(Z, g, s, mu, par, z);
     M3 W = mul(exp_alg8(z, c_exp), L);
     gram_schmidt(W);
     stu(Uout, g, s, mu, par, W);
+#endif
 }
 
 }  // namespace

@@ -98,12 +98,24 @@ __device__ __forceinline__ bool cb4_member(const int c[4], int mu, int colour)
     return ((c[0] + c[1] + c[2] + c[3] + c[mu]) & 3) == colour;
 }
 
+// overrelaxation with the software-pipelined staple (the heatbath's sampler
+// leaves no registers for it)
+#ifndef LT_OR_PIPE
+#define LT_OR_PIPE 1
+#endif
 template <bool HEATBATH>
 __device__ __forceinline__ void update_link(double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
                                             int par, double prefactor, uint64_t seed, uint64_t counter)
 {
-    M3 st = staple(U, g, s, c, mu);
-    M3 L = ldu(U, g, s, mu, par);
+    M3 st, L;
+    if (LT_OR_PIPE && !HEATBATH) {
+        Raw6 own;
+        st = staple_pipelined(U, g, s, c, mu, par, own);
+        L = su3_of(own);
+    } else {
+        st = staple(U, g, s, c, mu);
+        L = ldu(U, g, s, mu, par);
+    }
 #pragma unroll
     for (int sub = 0; sub < 3; ++sub) {
         const int i = sub == 2 ? 1 : 0;
