@@ -238,6 +238,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="skip the 16 x 16^4 PT-MetaD measurement")
     ap.add_argument("--no-48", action="store_true", help="skip the 48^4 measurement")
+    ap.add_argument("--pt-16-only", action="store_true",
+                    help="of the PT-MetaD measurements run only 16 x 16^4 (kernel profiles of that workload)")
     ap.add_argument("--eager", action="store_true",
                     help="PT-MetaD steps as eager launches instead of graph replays (kernel tracing)")
     args = ap.parse_args()
@@ -365,6 +367,7 @@ def main():
         g16 = (16, 16, 16, 16)
         result["pt_metad"] = pt_metad(la, ctx, [g16] * 16, graph=not args.eager)
         result["pt_metad"]["graph"] = not args.eager
+    if not args.no_pt and not args.pt_16_only:
         print("# PT-MetaD mixed 16 x 16^4 + 12^4 + 20^4 + 24^4, and 16 x 8^4", file=sys.stderr, flush=True)
         mixed = [g16] * 16 + [(12,) * 4, (20,) * 4, (24,) * 4]
         result["pt_metad_mixed"] = pt_metad(la, ctx, mixed, graph=not args.eager)

@@ -234,6 +234,29 @@ __device__ __forceinline__ M3 mul_nd_r01(const M3& a, const M3& b)
     r.m[6] = r.m[7] = r.m[8] = cd{0.0, 0.0};
     return r;
 }
+// rows 0 and 1 of a^+ b and of a^+ b^+ (a, b complete)
+__device__ __forceinline__ M3 mul_dn_r01(const M3& a, const M3& b)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            r.m[3 * i + j] = dot3<true, false>(a.m[i], b.m[j], a.m[3 + i], b.m[3 + j], a.m[6 + i], b.m[6 + j]);
+    r.m[6] = r.m[7] = r.m[8] = cd{0.0, 0.0};
+    return r;
+}
+__device__ __forceinline__ M3 mul_dd_r01(const M3& a, const M3& b)
+{
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            r.m[3 * i + j] = dot3<true, true>(a.m[i], b.m[3 * j], a.m[3 + i], b.m[3 * j + 1], a.m[6 + i], b.m[3 * j + 2]);
+    r.m[6] = r.m[7] = r.m[8] = cd{0.0, 0.0};
+    return r;
+}
 __device__ __forceinline__ M3 dag(const M3& a)
 {
     M3 r;
@@ -1060,25 +1083,27 @@ __device__ __forceinline__ M3 staple_pipelined(const double2* __restrict__ U, co
 __device__ __forceinline__ M3 clover_plane(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int par,
                                            int mu, int nu, double& retr_plaq)
 {
+    // every leaf is a product of four links, an SU(3) matrix: rows 0 and 1 are
+    // multiplied out along the chain, row 2 is conj(row0 x row1)
     const int q = par ^ 1;
     const int smu = up(g, s, c[mu], mu), snu = up(g, s, c[nu], nu);
     const int smd = dn(g, s, c[mu], mu), snd = dn(g, s, c[nu], nu);
     const int smd_nu = up(g, smd, c[nu], nu), smd_nd = dn(g, smd, c[nu], nu), smu_nd = up(g, snd, c[mu], mu);
     const M3 Umu = ldu(U, g, s, mu, par), Unu = ldu(U, g, s, nu, par);
     // leaf I: U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-    M3 l = mul_nd(mul_nd(mul(Umu, ldu(U, g, smu, nu, q)), ldu(U, g, snu, mu, q)), Unu);
-    retr_plaq = l.m[0].re + l.m[4].re + l.m[8].re;
-    M3 C = l;
+    M3 C = mul_nd_r01(mul_nd_r01(mul_r01(Umu, ldu(U, g, smu, nu, q)), ldu(U, g, snu, mu, q)), Unu);
+    third_row(C);
+    retr_plaq = C.m[0].re + C.m[4].re + C.m[8].re;
     // leaf II: U_nu(x) U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
     const M3 Umu_md = ldu(U, g, smd, mu, q);
-    l = mul(mul_nd(mul_nd(Unu, ldu(U, g, smd_nu, mu, par)), ldu(U, g, smd, nu, q)), Umu_md);
-    acc(C, l);
+    acc_su3_r01(C, mul_r01(mul_nd_r01(mul_nd_r01(Unu, ldu(U, g, smd_nu, mu, par)), ldu(U, g, smd, nu, q)), Umu_md));
     // leaf III: U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu)
-    l = mul(mul(mul_nd(dag(Umu_md), ldu(U, g, smd_nd, nu, par)), ldu(U, g, smd_nd, mu, par)), ldu(U, g, snd, nu, q));
-    acc(C, l);
+    acc_su3_r01(C, mul_r01(mul_r01(mul_dd_r01(Umu_md, ldu(U, g, smd_nd, nu, par)), ldu(U, g, smd_nd, mu, par)),
+                           ldu(U, g, snd, nu, q)));
     // leaf IV: U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^+
-    l = mul_nd(mul(mul_dn(ldu(U, g, snd, nu, q), ldu(U, g, snd, mu, q)), ldu(U, g, smu_nd, nu, par)), Umu);
-    acc(C, l);
+    acc_su3_r01(C, mul_nd_r01(mul_r01(mul_dn_r01(ldu(U, g, snd, nu, q), ldu(U, g, snd, mu, q)),
+                                      ldu(U, g, smu_nd, nu, par)),
+                              Umu));
     return C;
 }
 

@@ -72,14 +72,21 @@ __device__ __forceinline__ void pair_terms(const double2* __restrict__ U, const 
     q += retr_mul(Fa, Fb);
 }
 
-// grid (chunks per timeslice, Nt, 3 plane pairs): partial sums of the plaquette,
-// raw q and energy.  The pairs (01,23), (02,13), (03,12) enter
-// q = tr F01F23 - tr F02F13 + tr F03F12; one pair per thread keeps registers low.
-__global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, Geom g, int Vs, double* part)
+// grid: 3 plane pairs x chunks per timeslice x Nt blocks, pair fastest, spread
+// over the XCDs so each XCD covers a contiguous range of timeslices with all three
+// pairs of a chunk in neighbouring blocks (one L2-resident neighbourhood).  Partial
+// sums of the plaquette, raw q and energy go to part[(pair, t, chunk)].  The pairs
+// (01,23), (02,13), (03,12) enter q = tr F01F23 - tr F02F13 + tr F03F12; one pair
+// per thread keeps registers low.
+__global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, Geom g, int Vs, int nchunk,
+                                                 double* part)
 {
-    const int t = blockIdx.y;
-    const int r = blockIdx.x * BS + threadIdx.x;
-    const int pair = blockIdx.z;
+    int b = xcd_block(blockIdx.x, gridDim.x);
+    const int pair = b % 3;
+    b /= 3;
+    const int chunk = b % nchunk;
+    const int t = b / nchunk;
+    const int r = chunk * BS + threadIdx.x;
     double p = 0.0, q = 0.0, e = 0.0;
     if (r < Vs) {
         const int s = t * Vs + r;
@@ -95,7 +102,7 @@ __global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, 
             pair_terms<0, 3, 1, 2>(U, g, s, c, par, p, q, e);
         }
     }
-    block_sum3(p, q, e, part + 3 * (((size_t)pair * gridDim.y + t) * gridDim.x + blockIdx.x));
+    block_sum3(p, q, e, part + 3 * (((size_t)pair * g.L[0] + t) * nchunk + chunk));
 }
 
 // one thread per timeslice: ordered sum over pairs and chunks
@@ -127,7 +134,8 @@ void k_observables(lt_field u, double* host_out)
     const unsigned nchunk = nblocks(Vs, BS);
     double* part = ctx->scratch_for(9 * (size_t)nchunk * nt + 3 * (size_t)nt);
     double* fin = part + 9 * (size_t)nchunk * nt;
-    launch(ctx, "observables", obs_kernel, dim3(nchunk, nt, 3), dim3(BS), 0, (const double2*)u->d, g, Vs, part);
+    launch(ctx, "observables", obs_kernel, dim3(3 * nchunk * nt), dim3(BS), 0, (const double2*)u->d, g, Vs,
+           (int)nchunk, part);
     launch(ctx, "observables_finish", obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, (const double*)part,
            (int)nchunk, nt, fin);
     LT_HIP_CHECK(hipMemcpyAsync(host_out, fin, 3 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

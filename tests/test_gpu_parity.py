@@ -215,6 +215,20 @@ def test_flow(la, lto, gpu_ctx, integrator):
     fl.close()
 
 
+@pytest.mark.parametrize("shape", [(4, 6, 8, 16), (2, 4, 6, 32), (4, 4, 4, 12)])
+def test_flow_shapes(la, lto, gpu_ctx, shape):
+    """RK3 flow on extents that take the LDS-plane stage (Nz = 16, 32; 4 and 2 rows per
+    block, odd Nx) and the tiled stage (Nz = 12)."""
+    flush(f"flow rk3 {shape}")
+    U = smooth(lto, shape, 3)
+    f = field(la, gpu_ctx, shape, U)
+    g = field(la, gpu_ctx, shape)
+    fl = la.GradientFlowKernel(f, g, "rk3", 0.02)
+    fl(3)
+    assert rel_err(g.download(), lto.flow(U, shape, 0.02, 3, "rk3")) <= 1e-12
+    fl.close()
+
+
 def test_flow_golden16(la, lto, golden, gpu_ctx):
     flush("flow golden 16^4")
     U = lto.stout(lto.haar(G16, 0), G16, 0.1, 3)
