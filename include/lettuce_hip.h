@@ -35,6 +35,7 @@ extern "C" {
 #define LT_ENOMEM       -3   /* device allocation failed */
 #define LT_ESTATE       -4   /* object in the wrong state / mismatched shapes */
 #define LT_ENUMERIC     -5   /* non-finite or non-unitary links detected */
+#define LT_EIO          -6   /* file missing, unreadable, malformed or not writable */
 
 typedef struct lt_ctx_s   *lt_ctx;
 typedef struct lt_field_s *lt_field;
@@ -86,6 +87,23 @@ int lt_field_set_haar(lt_field f, uint64_t seed);
 int lt_field_reunitarize(lt_field f);
 /* max |U^+U - 1| and number of non-finite entries (SU3::Tests, su3.hpp:465-659) */
 int lt_field_check(lt_field f, double *max_unitarity_dev, long long *n_nonfinite);
+
+/* ---- BMW configuration files (IO/config_io/bmw_format.hpp) ----
+ * SaveConfigBMW (full_rows = 0, "#BMW", rows 0 and 1) and SaveConfigBMWFull
+ * (full_rows = 1, "#18BMW", all three rows) (bmw_format.hpp:315-402, 517-600): 4096-byte
+ * header "#BMW Nx Ny Nz Nt <checksum>", links in t, z, y, x order with directions
+ * x, y, z, t, big-endian doubles; an existing file is an error unless overwrite != 0.
+ * checksum_hex (hex_len bytes, may be null) receives the header's checksum. */
+int lt_field_save_bmw(lt_field f, const char *path, int full_rows, int overwrite, char *checksum_hex, int hex_len);
+/* LoadConfigBMW / LoadConfigBMWFull (bmw_format.hpp:153-313, 425-515): the header's
+ * extents must equal the field's; *checksum_ok tells whether the recomputed checksum
+ * equals the header's (a mismatch is reported, not an error, as in the reference);
+ * *max_deviation is max |1 - U^+U| (and, with three rows, the stored row 2 against
+ * conj(row0 x row1)); above 1e-6 (SU3::Tests::IsGroupElement) the call returns
+ * LT_ENUMERIC. */
+int lt_field_load_bmw(lt_field f, const char *path, int full_rows, int *checksum_ok, double *max_deviation);
+/* Checksum_Adler64 (bmw_format.hpp:55-100) of the field as it would be saved */
+int lt_field_bmw_checksum(lt_field f, int full_rows, unsigned long long *checksum);
 /* counter-based RNG state of the field (replaces PRNG4D, random/prng_wrapper.hpp:43-379) */
 int lt_field_set_rng(lt_field f, uint64_t seed, uint64_t counter);
 int lt_field_get_rng(lt_field f, uint64_t *seed, uint64_t *counter);
@@ -193,6 +211,18 @@ int lt_bias_derivative(lt_bias b, double cv, double *d);
 /* read (set = 0) or write (set = 1) the bins + 1 grid values / the current CV */
 int lt_bias_grid(lt_bias b, double *values, int n, int set);
 int lt_bias_cv(lt_bias b, double *cv, int set);
+
+/* MetaBiasPotential::SaveParameters / SavePotential / LoadPotential
+ * (metadynamics.hpp:304-505): the reference's text format (version line, parameter
+ * lines up to END_METADYN_PARAMS, comma-separated grid, exceeded_count); overwrite = 0
+ * appends.  Loading takes the parameters and the last grid snapshot of the file. */
+int lt_bias_save_parameters(lt_bias b, const char *path, int overwrite);
+int lt_bias_save_potential(lt_bias b, const char *path, int overwrite);
+int lt_bias_load_potential(lt_bias b, const char *path);
+/* the primary parameters (params[6] = CV_min, CV_max, weight, width,
+ * well_tempered_parameter, threshold_weight), bin_number and exceeded_count;
+ * any pointer may be null (no reference counterpart: its members are public) */
+int lt_bias_info(lt_bias b, double *params, int *bins, unsigned long long *exceeded);
 
 /* HMCMetaDKernel::operator() (hmc_metadynamics.hpp:552-655): MetaD-HMC trajectory with
  * the adjoint force dV/dQ * dQ/dU, path-sample bias updates when update_bias != 0;

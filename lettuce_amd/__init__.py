@@ -14,6 +14,8 @@ liblettuce_hip.so (include/lettuce_hip.h).  Names follow the reference:
   WilsonAction::Action (wilson_action.hpp:23-41)     WilsonAction.Action
   StoutSmearing4D / StoutSmearingN (stout_smearing)  StoutSmearing4D, StoutSmearingN
   GradientFlowKernel (smearing/gradient_flow.hpp)    GradientFlowKernel
+  SaveConfigBMW[Full] / LoadConfigBMW[Full]          same names (io.py; packed on the device)
+  CheckpointManager (checkpoint_manager.hpp)         CheckpointManager (io.py)
 
 Every call runs HIP kernels on the GPU; there is no CPU fallback.
 """
@@ -23,12 +25,15 @@ import numpy as np
 
 from . import _lib
 from ._lib import LettuceError, call
+from .io import (CheckpointManager, LoadConfigBMW, LoadConfigBMWFull, PhiloxState, SaveConfigBMW,  # noqa: F401
+                 SaveConfigBMWFull)
 
 __all__ = [
     "Context", "GaugeField", "Iterator", "HeatbathKernel", "OverrelaxationSubgroupKernel",
     "PlaquetteSum", "TopChargeClover", "TopChargeCloverTimeslice", "EnergyDensity", "WilsonAction",
     "StoutSmearing4D", "StoutSmearingN", "GradientFlowKernel", "Observables", "LettuceError",
-    "CB2", "CB4",
+    "CB2", "CB4", "SaveConfigBMW", "SaveConfigBMWFull", "LoadConfigBMW", "LoadConfigBMWFull", "CheckpointManager",
+    "PhiloxState",
 ]
 
 CB2 = 2
@@ -166,6 +171,28 @@ class GaugeField:
         c = ctypes.c_uint64()
         call("lt_field_get_rng", self._h, ctypes.byref(s), ctypes.byref(c))
         return s.value, c.value
+
+    def save_bmw(self, path, full=False, overwrite=False):
+        """Write the field as a BMW file ("#BMW", rows 0-1; full: "#18BMW", all rows),
+        packed on the device; returns the header's checksum (hex)."""
+        buf = ctypes.create_string_buffer(32)
+        call("lt_field_save_bmw", self._h, str(path).encode(), 1 if full else 0, 1 if overwrite else 0, buf, 32)
+        return buf.value.decode()
+
+    def load_bmw(self, path, full=False):
+        """Read a BMW file into the field; returns (checksum_ok, max deviation from SU(3)).
+        Raises LettuceError for a missing / malformed file, other extents, or links that
+        are not SU(3) within 1e-6."""
+        ok = ctypes.c_int()
+        dev = ctypes.c_double()
+        call("lt_field_load_bmw", self._h, str(path).encode(), 1 if full else 0, ctypes.byref(ok), ctypes.byref(dev))
+        return bool(ok.value), dev.value
+
+    def bmw_checksum(self, full=False):
+        """Checksum_Adler64 of the field as save_bmw would write it (an integer)."""
+        v = ctypes.c_ulonglong()
+        call("lt_field_bmw_checksum", self._h, 1 if full else 0, ctypes.byref(v))
+        return v.value
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -460,6 +487,33 @@ class MetaBiasPotential:
     def set_grid(self, values):
         a = np.ascontiguousarray(values, dtype=np.float64)
         call("lt_bias_grid", self._h, _dp(a), self.bins + 1, 1)
+
+    def SaveParameters(self, filename, overwrite=False):
+        """MetaBiasPotential::SaveParameters (metadynamics.hpp:304-324); appends unless overwrite."""
+        call("lt_bias_save_parameters", self._h, str(filename).encode(), 1 if overwrite else 0)
+
+    def SavePotential(self, filename, overwrite=False):
+        """MetaBiasPotential::SavePotential (metadynamics.hpp:326-346); appends unless overwrite."""
+        call("lt_bias_save_potential", self._h, str(filename).encode(), 1 if overwrite else 0)
+
+    def LoadPotential(self, filename):
+        """MetaBiasPotential::LoadPotential (metadynamics.hpp:351-505): parameters and the
+        last potential snapshot of the file.  Returns False (and leaves the potential
+        unchanged) where the reference returns false."""
+        try:
+            call("lt_bias_load_potential", self._h, str(filename).encode())
+        except LettuceError:
+            return False
+        self.bins = self.info()["bin_number"]
+        return True
+
+    def info(self):
+        p = np.zeros(6)
+        n = ctypes.c_int()
+        e = ctypes.c_ulonglong()
+        call("lt_bias_info", self._h, _dp(p), ctypes.byref(n), ctypes.byref(e))
+        return {"CV_min": p[0], "CV_max": p[1], "weight": p[2], "width": p[3], "well_tempered_parameter": p[4],
+                "threshold_weight": p[5], "bin_number": n.value, "exceeded_count": e.value}
 
     def ReturnCV_current(self):
         v = ctypes.c_double()

@@ -92,6 +92,13 @@ SIGNATURES = {
     "lt_batch_samples": [ctypes.c_void_p, D, I],
     "lt_batch_swap_stats": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), I],
     "lt_autotune_field": [c_field, I, ctypes.c_char_p, ctypes.POINTER(I), D],
+    "lt_field_save_bmw": [c_field, ctypes.c_char_p, I, I, ctypes.c_char_p, I],
+    "lt_field_load_bmw": [c_field, ctypes.c_char_p, I, ctypes.POINTER(I), D],
+    "lt_field_bmw_checksum": [c_field, I, ctypes.POINTER(ctypes.c_ulonglong)],
+    "lt_bias_save_parameters": [ctypes.c_void_p, ctypes.c_char_p, I],
+    "lt_bias_save_potential": [ctypes.c_void_p, ctypes.c_char_p, I],
+    "lt_bias_load_potential": [ctypes.c_void_p, ctypes.c_char_p],
+    "lt_bias_info": [ctypes.c_void_p, D, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_ulonglong)],
     "lt_batch_autotune": [ctypes.c_void_p, ctypes.c_char_p],
 }
 

@@ -210,6 +210,11 @@ void k_staple_field(lt_field u, lt_field out);
 void k_or_sweep(lt_field u, int n, int schedule);
 void k_hb_sweep(lt_field u, double beta, int n, int schedule);
 // fused observables: per-timeslice plaquette sum, raw q, E_clov numerator -> host[3*Nt]
+// BMW sites [n0, n0 + count) <-> big-endian file words (4 * K per site, K = 12 or
+// 18), with per-block checksum partials (A, B) and, on unpacking, row-2 deviations
+void k_bmw_pack(lt_field f, int K, long long n0, long long count, uint64_t* d_out, uint64_t* d_part);
+void k_bmw_unpack(lt_field f, int K, long long n0, long long count, const uint64_t* d_in, uint64_t* d_part,
+                  double* d_dev);
 void k_observables(lt_field u, double* host_out);
 void k_stout(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double rho);
 void k_flow_stage(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double2* z, double c_old,

@@ -45,6 +45,7 @@ void   lto_reunitarize(double *U, lto_dims d);
 /* ---- SU(3) helpers exposed for tests ------------------------------------- */
 void   lto_su3_exp(const double *Q, double *out);           /* exp(iQ), Q traceless hermitian */
 void   lto_gram_schmidt(double *M);
+void   lto_third_row_fma(double *U, size_t nlinks);
 void   lto_algebra(const double *M, double *out);           /* traceless anti-hermitian part */
 
 /* ---- actions / updates --------------------------------------------------- */

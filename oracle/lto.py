@@ -66,6 +66,7 @@ def _load():
         "lto_gauge_force_update": (None, [D, D, dims, ctypes.c_double, ctypes.c_double]),
         "lto_field_update": (None, [D, D, dims, ctypes.c_double]),
         "lto_kinetic": (ctypes.c_double, [D, dims]),
+        "lto_third_row_fma": (None, [D, ctypes.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -396,3 +397,72 @@ def ladder_exchange(labels, strengths, cvs, shapes, bias, pt_seed, pt_counter, m
         if acc:
             labels[i], labels[j] = k + 1, k
     return labels, out
+
+
+# ---------------------------------------------------------------------------
+# BMW configuration files (LettuceGauge/IO/config_io/bmw_format.hpp) and the
+# metadynamics bias-potential text file (actions/bias_potential/metadynamics.hpp)
+
+ADLER64_PRIME = 4294967291
+
+
+def third_row_fma(U):
+    """Copy of U with row 2 rebuilt from rows 0 and 1 as the device does."""
+    V = np.ascontiguousarray(U, dtype=np.float64).copy()
+    lib().lto_third_row_fma(_p(V), V.size // 18)
+    return V
+
+
+def bmw_words(U, shape, full):
+    """Native 64-bit words of the link data in file order: sites t, z, y, x (slowest
+    to fastest), directions x, y, z, t, rows 0-1 (or 0-2) row-major (re, im)
+    (bmw_format.hpp:29-49, 220-260).  Shape (V, 4 * K)."""
+    nt, nx, ny, nz = shape
+    K = 18 if full else 12
+    A = np.asarray(U, dtype=np.float64).reshape(nt, nx, ny, nz, 4, 18)[..., :K]
+    A = A.transpose(0, 3, 2, 1, 4, 5)[:, :, :, :, [1, 2, 3, 0], :]
+    return np.ascontiguousarray(A).reshape(-1, 4 * K).view(np.uint64)
+
+
+def bmw_checksum(words):
+    """Checksum_Adler64 (bmw_format.hpp:55-100) over blocks = rows of `words`:
+    word i of block n has r = (N - n) * 2 * B - 2 i (B words per block, N blocks);
+    A += h + l, B += (2 + r h + (r - 1) l) mod p with 64-bit wrapping products;
+    final = ((B mod p) << 32) + (A + 1) mod p."""
+    W = np.asarray(words, dtype=np.uint64)
+    nblk, bsz = W.shape
+    p = np.uint64(ADLER64_PRIME)
+    n = np.arange(nblk, dtype=np.uint64)[:, None]
+    i = np.arange(bsz, dtype=np.uint64)[None, :]
+    r = (np.uint64(nblk) - n) * np.uint64(2 * bsz) - np.uint64(2) * i
+    h = W >> np.uint64(32)
+    l = W & np.uint64(0xFFFFFFFF)
+    with np.errstate(over="ignore"):
+        b = (np.uint64(2) + r * h + (r - np.uint64(1)) * l) % p
+    A = int((h + l).sum(dtype=np.uint64)) % ADLER64_PRIME
+    Bs = int(b.sum(axis=1, dtype=np.uint64).astype(object).sum()) % ADLER64_PRIME
+    return (Bs << 32) + (A + 1) % ADLER64_PRIME
+
+
+def bmw_file_bytes(U, shape, full):
+    """The complete file SaveConfigBMW / SaveConfigBMWFull write for U: the 4096-byte
+    header ("#BMW Nx Ny Nz Nt <hex checksum>", a comment line, the NUL that snprintf
+    leaves, newlines) and the big-endian link data."""
+    nt, nx, ny, nz = shape
+    W = bmw_words(U, shape, full)
+    cs = "%x" % bmw_checksum(W)
+    first = "#%sBMW %d %d %d %d %s\n" % ("18" if full else "", nx, ny, nz, nt, cs)
+    head = (first + "\nGenerated with Lettuce\n").encode() + b"\0"
+    head = head + b"\n" * (4096 - len(head))
+    return head + W.astype(">u8").tobytes(), cs
+
+
+def bias_file_text(cv_min, cv_max, bins, weight, width, wt_param, threshold, grid, exceeded):
+    """MetaBiasPotential::SaveParameters followed by SavePotential (metadynamics.hpp:
+    304-346): values in the default stream format (six significant digits, %g)."""
+    g = lambda v: "%g" % v
+    lines = ["SU(3)_version_1.3", "Metadynamics Potential", "CV_min: " + g(cv_min), "CV_max: " + g(cv_max),
+             "bin_number: %d" % bins, "weight: " + g(weight), "width: " + g(width),
+             "well_tempered_parameter: " + g(wt_param), "threshold_weight: " + g(threshold), "END_METADYN_PARAMS",
+             ",".join(g(v) for v in grid), "exceeded_count: %d" % exceeded]
+    return "\n".join(lines) + "\n"

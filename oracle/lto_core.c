@@ -103,6 +103,35 @@ void lto_reunitarize(double *U, lto_dims d)
 }
 
 void lto_gram_schmidt(double *M) { mat3 a = m_load(M); m_gram_schmidt(&a); m_store(M, a); }
+
+/* conj(x y - z w) with the rounding steps of the device (one product, then three
+ * fused multiply-adds per part) */
+static void conj_cross_fma(const double *x, const double *y, const double *z, const double *w, double *out)
+{
+    double re = z[1] * w[1];
+    re = fma(-z[0], w[0], re);
+    re = fma(-x[1], y[1], re);
+    re = fma(x[0], y[0], re);
+    double im = z[1] * w[0];
+    im = fma(z[0], w[1], im);
+    im = fma(-x[1], y[0], im);
+    im = fma(-x[0], y[1], im);
+    out[0] = re;
+    out[1] = im;
+}
+
+/* Row 2 of every link rebuilt from rows 0 and 1 as conj(row0 x row1)
+ * (SU3::Projection::RestoreLastRow, math/su3.hpp:362-367) with the device's rounding:
+ * the links a device field holds after an upload of rows 0 and 1. */
+void lto_third_row_fma(double *U, size_t nlinks)
+{
+    for (size_t i = 0; i < nlinks; ++i) {
+        double *m = U + 18 * i;   /* entry k at m + 2k, row-major */
+        conj_cross_fma(m + 2, m + 10, m + 4, m + 8, m + 12);
+        conj_cross_fma(m + 4, m + 6, m + 0, m + 10, m + 14);
+        conj_cross_fma(m + 0, m + 8, m + 2, m + 6, m + 16);
+    }
+}
 void lto_algebra(const double *M, double *out) { m_store(out, m_algebra(m_load(M))); }
 
 /* ------------------------------------------------------------------------- */

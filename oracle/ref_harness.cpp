@@ -29,6 +29,7 @@
 #include "LettuceGauge/updates/hmc_gauge.hpp"
 #include "LettuceGauge/updates/hmc_metadynamics.hpp"
 #include "LettuceGauge/updates/tempering.hpp"
+#include "LettuceGauge/IO/config_io/bmw_format.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -410,6 +411,24 @@ int main(int argc, char** argv)
         GradientFlowKernel Flow(*U, *U2, I, GaugeAction::WilsonAction, 0.01);
         double tfl = time_it([&] { Flow(nrep); }) / nrep;
         std::printf("hb_sweep_s %.6g\nor_sweep_s %.6g\nrk3_step_s %.6g\nthreads %d\n", thb, tor, tfl, omp_get_max_threads());
+    } else if (cmd == "bmw_save") {              // bmw_save in out full(0|1): the reference's BMW writer
+        read_field(argv[2], *U);
+        const bool ok = std::atoi(argv[4]) ? SaveConfigBMWFull(*U, argv[3], true) : SaveConfigBMW(*U, argv[3], true);
+        if (!ok) return 3;
+    } else if (cmd == "bmw_load") {              // bmw_load file out full(0|1): the reference's BMW reader
+        const bool ok = std::atoi(argv[4]) ? LoadConfigBMWFull(*U, argv[2]) : LoadConfigBMW(*U, argv[2]);
+        write_field(argv[3], *U);
+        std::printf("load_ok %d\n", (int)ok);
+    } else if (cmd == "bias_save") {             // bias_save out cvmin cvmax bins weight width wt thr ndep dep...
+        MetaBiasPotential B(std::atof(argv[3]), std::atof(argv[4]), std::atoi(argv[5]), std::atof(argv[6]),
+                            std::atof(argv[7]), std::atof(argv[8]), std::atof(argv[9]));
+        int k = 10;
+        int ndep = std::atoi(argv[k++]);
+        std::vector<double> deps;
+        for (int i = 0; i < ndep; ++i) deps.push_back(std::atof(argv[k++]));
+        B.UpdatePotentialSymmetric(deps);
+        B.SaveParameters(argv[2], true);
+        B.SavePotential(argv[2]);
     } else {
         std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
         return 2;
