@@ -5,6 +5,9 @@
 
 #include <array>
 #include <cstdint>
+#include <filesystem>
+#include <fstream>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -194,6 +197,18 @@ class MetaBiasPotential {
         LETTUCE_HIP_CALL(lt_bias_derivative(h_, cv, &v));
         return v;
     }
+    // SaveParameters / SavePotential / LoadPotential (metadynamics.hpp:304-505): the
+    // reference's text files; saving appends unless overwrite, loading returns false on
+    // a missing or foreign file
+    void SaveParameters(const std::string& filename, bool overwrite = false) const
+    {
+        LETTUCE_HIP_CALL(lt_bias_save_parameters(h_, filename.c_str(), overwrite ? 1 : 0));
+    }
+    void SavePotential(const std::string& filename, bool overwrite = false) const
+    {
+        LETTUCE_HIP_CALL(lt_bias_save_potential(h_, filename.c_str(), overwrite ? 1 : 0));
+    }
+    bool LoadPotential(const std::string& filename) { return lt_bias_load_potential(h_, filename.c_str()) == LT_OK; }
     lt_bias handle() const { return h_; }
 
   private:
@@ -308,6 +323,120 @@ class ReplicaBatch {
     }
     lt_batch h_ = nullptr;
     int n_ = 0;
+};
+
+// SaveConfigBMW / SaveConfigBMWFull / LoadConfigBMW / LoadConfigBMWFull
+// (IO/config_io/bmw_format.hpp): packed and unpacked on the device; bool results
+// as in the reference (false: file exists without overwrite, missing or foreign
+// file, other extents, links not in SU(3) within 1e-6)
+inline bool SaveConfigBMW(const GaugeField& U, const std::string& filename, bool overwrite = false)
+{
+    return lt_field_save_bmw(U.handle(), filename.c_str(), 0, overwrite ? 1 : 0, nullptr, 0) == LT_OK;
+}
+inline bool SaveConfigBMWFull(const GaugeField& U, const std::string& filename, bool overwrite = false)
+{
+    return lt_field_save_bmw(U.handle(), filename.c_str(), 1, overwrite ? 1 : 0, nullptr, 0) == LT_OK;
+}
+inline bool LoadConfigBMW(GaugeField& U, const std::string& filename)
+{
+    int ok = 0;
+    double dev = 0.0;
+    return lt_field_load_bmw(U.handle(), filename.c_str(), 0, &ok, &dev) == LT_OK;
+}
+inline bool LoadConfigBMWFull(GaugeField& U, const std::string& filename)
+{
+    int ok = 0;
+    double dev = 0.0;
+    return lt_field_load_bmw(U.handle(), filename.c_str(), 1, &ok, &dev) == LT_OK;
+}
+
+// Generator state of a field: seed and counter of its Philox stream, in place of
+// the PCG state files of PRNG4D::SaveState / LoadState; the second file (the
+// normal distribution's cached value there) only carries a marker, the Gaussian
+// draws here keep no state
+struct PhiloxState {
+    GaugeField& U;
+    bool SaveState(const std::filesystem::path& prng_file, const std::filesystem::path& distribution_file,
+                   bool overwrite = false) const
+    {
+        if (!overwrite && (std::filesystem::exists(prng_file) || std::filesystem::exists(distribution_file)))
+            return false;
+        std::uint64_t seed = 0, counter = 0;
+        LETTUCE_HIP_CALL(lt_field_get_rng(U.handle(), &seed, &counter));
+        std::ofstream(prng_file) << "philox4x32-10 " << seed << " " << counter << "\n";
+        std::ofstream(distribution_file) << "philox4x32-10 stateless\n";
+        return true;
+    }
+    bool LoadState(const std::filesystem::path& prng_file, const std::filesystem::path& = {}) const
+    {
+        std::ifstream in(prng_file);
+        std::string tag;
+        std::uint64_t seed = 0, counter = 0;
+        if (!(in >> tag >> seed >> counter) || tag != "philox4x32-10") return false;
+        LETTUCE_HIP_CALL(lt_field_set_rng(U.handle(), seed, counter));
+        return true;
+    }
+};
+
+// CheckpointManager (IO/config_io/checkpoint_manager.hpp:28-159): the newest
+// checkpoint in the directory, older generations in backup1, backup2, ... (higher
+// is older), n_checkpoint_files generations kept
+class CheckpointManager {
+  public:
+    CheckpointManager(std::filesystem::path dir, int n_checkpoint_files, bool create_subdirectories)
+        : dir_(std::move(dir)), n_(n_checkpoint_files)
+    {
+        if (create_subdirectories) CreateSubdirectories();
+    }
+    const std::filesystem::path& CheckpointDirectory() const { return dir_; }
+    static std::string ReturnBackupSubdirectory(int k) { return k == 0 ? "" : "backup" + std::to_string(k); }
+    void CreateSubdirectories() const
+    {
+        for (int k = 1; k < n_; ++k) std::filesystem::create_directories(dir_ / ReturnBackupSubdirectory(k));
+    }
+    static void RotateFiles(const std::filesystem::path& current, const std::filesystem::path& old, bool copy = true)
+    {
+        if (!std::filesystem::exists(current)) return;
+        std::error_code err;
+        if (copy)
+            std::filesystem::copy_file(current, old, std::filesystem::copy_options::overwrite_existing, err);
+        else
+            std::filesystem::rename(current, old, err);
+    }
+    template <typename SaveT>
+    void AlternatingConfigCheckpoints(SaveT&& save, const GaugeField& U, const std::string& config) const
+    {
+        Rotate({config});
+        save(U, (dir_ / config).string(), true);
+    }
+    template <typename SaveT, typename PrngT>
+    void AlternatingCheckpoints(SaveT&& save, PrngT& prng, const GaugeField& U, const std::string& config,
+                                const std::string& prng_file, const std::string& distribution_file) const
+    {
+        Rotate({config, prng_file, distribution_file});
+        save(U, (dir_ / config).string(), true);
+        prng.SaveState(dir_ / prng_file, dir_ / distribution_file, true);
+    }
+    template <typename LoadT, typename PrngT>
+    bool LoadCheckpoint(LoadT&& load, PrngT& prng, GaugeField& U, const std::string& config,
+                        const std::string& prng_file, const std::string& distribution_file) const
+    {
+        const bool ok = load(U, (dir_ / config).string());
+        return prng.LoadState(dir_ / prng_file, dir_ / distribution_file) && ok;
+    }
+    static void CreateCompletedFile(const std::filesystem::path& where) { std::ofstream(where / "completed_run"); }
+
+  private:
+    // generation k-1 -> k from the oldest down; the newest is moved, not copied
+    void Rotate(std::initializer_list<std::string> names) const
+    {
+        for (int k = n_ - 1; k > 0; --k)
+            for (const auto& name : names)
+                RotateFiles(dir_ / ReturnBackupSubdirectory(k - 1) / name, dir_ / ReturnBackupSubdirectory(k) / name,
+                            k != 1);
+    }
+    std::filesystem::path dir_;
+    int n_;
 };
 
 }  // namespace lettuce_hip

@@ -36,5 +36,16 @@ int main()
     Temper.Step(10, 6.0, LT_INT_OMF2, 1.0, true, LT_SWAP_EVEN);
     std::printf("MetaD accepted %d dH %.6f labels %d %d\n", (int)accepted, HMC_MetaD.delta_H, Temper.Labels()[0],
                 Temper.Labels()[1]);
+
+    // rotating checkpoints and the bias file (SU3_main.cpp:287-294, 503-504, 631)
+    CheckpointManager Checkpointer("checkpoints", 3, true);
+    PhiloxState prng{U};
+    Checkpointer.AlternatingCheckpoints(SaveConfigBMWFull, prng, U, "config.conf", "prng_state.txt",
+                                        "distribution_state.txt");
+    Bias.SaveParameters("metapotential.txt", true);
+    Bias.SavePotential("metapotential.txt");
+    const bool resumed = Checkpointer.LoadCheckpoint(LoadConfigBMWFull, prng, U, "config.conf", "prng_state.txt",
+                                                     "distribution_state.txt");
+    std::printf("resumed %d\n", (int)resumed);
     return 0;
 }
