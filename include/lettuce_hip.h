@@ -129,6 +129,19 @@ int lt_heatbath_sweep(lt_field u, double beta, int n_sweep, int schedule);
 int lt_plaquette_sum(lt_field u, double *out);
 /* WilsonAction::Action / Rectangular<1>::Action (wilson_action.hpp:23-41) */
 int lt_wilson_action(lt_field u, double beta, double *out);
+/* Rectangular<2>::ActionNormalized (rectangular_action.hpp:61-100): plaquettes and 1x2 +
+ * 2x1 rectangles, (c_plaq (6V - sum Re tr P / 3) + c_rect (12V - sum Re tr R / 3)) / (6V);
+ * tree-level Symanzik: c_plaq = 1 + 8/12, c_rect = -1/12 (SU3_main.cpp:119) */
+int lt_action_rectangular(lt_field u, double c_plaq, double c_rect, double *action_normalized);
+/* TopChargePlaquette (topological_charge.hpp:366-416): single-plaquette field strength */
+int lt_topcharge_plaquette(lt_field u, double *q);
+/* WilsonLoop<n_start, n_end, reset>(U, U_chain) (wilson_loop.hpp:65-126): the matrix
+ * field `chain` holds the straight link products between calls (reset = 1 starts them
+ * from the unit matrix); returns 1 - sum Re tr W / (18 V) over the six planes of the
+ * n_end x n_end loops */
+int lt_wilson_loop(lt_field u, lt_field chain, int n_start, int n_end, int reset, double *w);
+/* PolyakovLoop (polyakov_loop.hpp:31-49): spatial average of tr prod_t U_t */
+int lt_polyakov_loop(lt_field u, double *re, double *im);
 /* TopChargeClover (topological_charge.hpp:97-231); q_t (Nt entries, may be NULL)
  * as TopChargeCloverTimeslice (:337-364) */
 int lt_topcharge_clover(lt_field u, double *Q, double *q_t);

@@ -33,7 +33,7 @@ __all__ = [
     "PlaquetteSum", "TopChargeClover", "TopChargeCloverTimeslice", "EnergyDensity", "WilsonAction",
     "StoutSmearing4D", "StoutSmearingN", "GradientFlowKernel", "Observables", "LettuceError",
     "CB2", "CB4", "SaveConfigBMW", "SaveConfigBMWFull", "LoadConfigBMW", "LoadConfigBMWFull", "CheckpointManager",
-    "PhiloxState",
+    "PhiloxState", "RectangularAction", "TopChargePlaquette", "WilsonLoop", "PolyakovLoop",
 ]
 
 CB2 = 2
@@ -296,6 +296,45 @@ class EnergyDensity:
         et = np.zeros(U.shape[0])
         call("lt_energy", U.handle, None, None, _dp(et))
         return et
+
+
+class RectangularAction:
+    """GaugeAction::Rectangular<2>(beta, c_plaq, c_rect) (rectangular_action.hpp:29-100);
+    ActionNormalized does not depend on beta.  Symanzik: c_plaq = 1 + 8/12, c_rect = -1/12."""
+
+    def __init__(self, beta, c_plaq, c_rect):
+        self.beta, self.c_plaq, self.c_rect = beta, c_plaq, c_rect
+
+    def ActionNormalized(self, U):
+        v = ctypes.c_double()
+        call("lt_action_rectangular", U.handle, self.c_plaq, self.c_rect, ctypes.byref(v))
+        return v.value
+
+    def Action(self, U):
+        return self.ActionNormalized(U) * 6.0 * self.beta * U.volume
+
+
+def TopChargePlaquette(U):
+    """TopChargePlaquette (topological_charge.hpp:366-416)."""
+    v = ctypes.c_double()
+    call("lt_topcharge_plaquette", U.handle, ctypes.byref(v))
+    return v.value
+
+
+def WilsonLoop(n_start, n_end, reset, U, U_chain):
+    """WilsonLoop<n_start, n_end, reset>(U, U_chain) (wilson_loop.hpp:65-126); U_chain is a
+    MatrixField that keeps the link chains between calls."""
+    v = ctypes.c_double()
+    call("lt_wilson_loop", U.handle, U_chain.handle, n_start, n_end, 1 if reset else 0, ctypes.byref(v))
+    return v.value
+
+
+def PolyakovLoop(U):
+    """PolyakovLoop (polyakov_loop.hpp:31-49), complex."""
+    re = ctypes.c_double()
+    im = ctypes.c_double()
+    call("lt_polyakov_loop", U.handle, ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)
 
 
 class WilsonAction:

@@ -544,6 +544,58 @@ int lt_observables(lt_field u, double* out)
     });
 }
 
+int lt_action_rectangular(lt_field u, double c_plaq, double c_rect, double* action_normalized)
+{
+    if (!u || !action_normalized) return fail(LT_EINVAL, "null argument");
+    if (!u->su3) return fail(LT_ESTATE, "lt_action_rectangular: the field must hold SU(3) links");
+    return guarded([&] {
+        use_device(u->ctx);
+        double r[3];
+        lt::k_rect_sums(u, r);
+        const double V = (double)u->g.V;
+        *action_normalized = (c_plaq * (6.0 * V - r[0] / 3.0) + c_rect * (12.0 * V - r[1] / 3.0)) / (6.0 * V);
+        return LT_OK;
+    });
+}
+
+int lt_topcharge_plaquette(lt_field u, double* q)
+{
+    if (!u || !q) return fail(LT_EINVAL, "null argument");
+    if (!u->su3) return fail(LT_ESTATE, "lt_topcharge_plaquette: the field must hold SU(3) links");
+    return guarded([&] {
+        use_device(u->ctx);
+        double r[3];
+        lt::k_rect_sums(u, r);
+        *q = r[2] / (16.0 * M_PI * M_PI);
+        return LT_OK;
+    });
+}
+
+int lt_wilson_loop(lt_field u, lt_field chain, int n_start, int n_end, int reset, double* w)
+{
+    if (!u || !chain || !w) return fail(LT_EINVAL, "null argument");
+    if (n_start < 0 || n_end <= n_start) return fail(LT_EINVAL, "lt_wilson_loop: 0 <= n_start < n_end required");
+    if (!u->su3 || chain->su3) return fail(LT_ESTATE, "lt_wilson_loop: SU(3) field and matrix-field chain required");
+    if (!same_shape(u, chain) || u->ctx != chain->ctx) return fail(LT_ESTATE, "lt_wilson_loop: shape/context mismatch");
+    return guarded([&] {
+        use_device(u->ctx);
+        const double s = lt::k_wilson_loop(u, chain, n_start, n_end, reset);
+        *w = 1.0 - s / (18.0 * (double)u->g.V);
+        return LT_OK;
+    });
+}
+
+int lt_polyakov_loop(lt_field u, double* re, double* im)
+{
+    if (!u || !re || !im) return fail(LT_EINVAL, "null argument");
+    if (!u->su3) return fail(LT_ESTATE, "lt_polyakov_loop: the field must hold SU(3) links");
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_polyakov_loop(u, re, im);
+        return LT_OK;
+    });
+}
+
 int lt_plaquette_sum(lt_field u, double* out)
 {
     if (!u || !out) return fail(LT_EINVAL, "null argument");

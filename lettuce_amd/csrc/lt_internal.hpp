@@ -216,6 +216,9 @@ void k_bmw_pack(lt_field f, int K, long long n0, long long count, uint64_t* d_ou
 void k_bmw_unpack(lt_field f, int K, long long n0, long long count, const uint64_t* d_in, uint64_t* d_part,
                   double* d_dev);
 void k_observables(lt_field u, double* host_out);
+void k_rect_sums(lt_field u, double out[3]);
+double k_wilson_loop(lt_field u, lt_field chain, int n0, int n1, int reset);
+void k_polyakov_loop(lt_field u, double* re, double* im);
 void k_stout(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double rho);
 void k_flow_stage(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double2* z, double c_old,
                   double c_new, double c_exp, int read_z, int write_z);

@@ -123,7 +123,221 @@ __global__ void obs_finish_kernel(const double* part, int nchunk, int nt, double
     out[3 * t + 2] = c;
 }
 
+// ---- the rest of Observables() (SU3_main.cpp:125-136) --------------------------
+
+// neighbour of lexicographic site s with coordinates c, n steps along mu (|n| < L)
+__device__ __forceinline__ int shift(const Geom& g, int s, const int c[4], int mu, int n)
+{
+    int v = c[mu] + n;
+    v = v >= g.L[mu] ? v - g.L[mu] : (v < 0 ? v + g.L[mu] : v);
+    return s + (v - c[mu]) * g.st[mu];
+}
+__device__ __forceinline__ M3 link(const double2* __restrict__ U, const Geom& g, int s, int mu)
+{
+    int c[4];
+    coords_of(g, s, c);
+    return ldu(U, g, s, mu, parity_of(c));
+}
+
+// plaquette P_munu(x) (plaquette.hpp:20-24) and Re tr of the 1x2 and 2x1 loops
+// (wilson_loop.hpp:155-175): each loop as Re tr(A B^+) of its two 3-link halves
+__device__ __forceinline__ void plane_loops(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
+                                            int nu, M3& P, double& rect)
+{
+    const int smu = shift(g, s, c, mu, 1), snu = shift(g, s, c, nu, 1);
+    const M3 Umu = link(U, g, s, mu), Unu = link(U, g, s, nu);
+    const M3 Unu_mu = link(U, g, smu, nu), Umu_nu = link(U, g, snu, mu);
+    P = mul_nd_r01(mul_nd_r01(mul_r01(Umu, Unu_mu), Umu_nu), Unu);
+    third_row(P);
+    int cm[4] = {c[0], c[1], c[2], c[3]}, cn[4] = {c[0], c[1], c[2], c[3]};
+    cm[mu] = cm[mu] + 1 == g.L[mu] ? 0 : cm[mu] + 1;
+    cn[nu] = cn[nu] + 1 == g.L[nu] ? 0 : cn[nu] + 1;
+    // 1x2: U_mu(x) U_nu(x+mu) U_nu(x+mu+nu)  vs  U_nu(x) U_nu(x+nu) U_mu(x+2nu)
+    {
+        M3 a = mul_r01(Umu, Unu_mu);
+        third_row(a);
+        a = mul_r01(a, link(U, g, shift(g, smu, cm, nu, 1), nu));
+        third_row(a);
+        M3 b = mul_r01(Unu, link(U, g, snu, nu));
+        third_row(b);
+        b = mul_r01(b, link(U, g, shift(g, snu, cn, nu, 1), mu));
+        third_row(b);
+        rect = retr_mul(a, dag(b));
+    }
+    // 2x1: U_mu(x) U_mu(x+mu) U_nu(x+2mu)  vs  U_nu(x) U_mu(x+nu) U_mu(x+mu+nu)
+    {
+        M3 a = mul_r01(Umu, link(U, g, smu, mu));
+        third_row(a);
+        a = mul_r01(a, link(U, g, shift(g, smu, cm, mu, 1), nu));
+        third_row(a);
+        M3 b = mul_r01(Unu, Umu_nu);
+        third_row(b);
+        b = mul_r01(b, link(U, g, shift(g, snu, cn, mu, 1), mu));
+        third_row(b);
+        rect += retr_mul(a, dag(b));
+    }
+}
+
+__device__ __forceinline__ void block_sum2(double v0, double v1, double* out)
+{
+    __shared__ double sh[2][BS];
+    sh[0][threadIdx.x] = v0;
+    sh[1][threadIdx.x] = v1;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+    }
+}
+
+// grid: 3 plane pairs x ceil(V / BS) blocks, pair fastest, XCD-remapped.  Per block:
+// sum Re tr P, sum Re tr (R12 + R21) and the plaquette charge density
+// Re tr (F_a F_b) (signed as in q = F01F23 + F02F31 + F03F12, F = -i (P - P^+))
+__global__ void __launch_bounds__(BS) rect_kernel(const double2* __restrict__ U, Geom g, double* part)
+{
+    int b = xcd_block(blockIdx.x, gridDim.x);
+    const int pair = b % 3, chunk = b / 3;
+    const int s = chunk * BS + threadIdx.x;
+    double p = 0.0, r = 0.0, q = 0.0;
+    if (s < g.V) {
+        int c[4];
+        coords_of(g, s, c);
+        // pairs (01, 23), (02, 13) with F31 = -F13, (03, 12)
+        const int a0 = 0, a1 = pair + 1;
+        const int b0 = pair == 2 ? 1 : (pair == 0 ? 2 : 1), b1 = pair == 0 ? 3 : (pair == 1 ? 3 : 2);
+        M3 Pa, Pb;
+        double ra, rb;
+        plane_loops(U, g, s, c, a0, a1, Pa, ra);
+        plane_loops(U, g, s, c, b0, b1, Pb, rb);
+        p = Pa.m[0].re + Pa.m[4].re + Pa.m[8].re + Pb.m[0].re + Pb.m[4].re + Pb.m[8].re;
+        r = ra + rb;
+        // F = -i (P - P^+); Re tr (Fa Fb) = -Re tr ((Pa - Pa^+)(Pb - Pb^+))
+        const M3 Da = sub(Pa, dag(Pa)), Db = sub(Pb, dag(Pb));
+        q = -retr_mul(Da, Db);
+        if (pair == 1) q = -q;
+    }
+    block_sum3(p, r, q, part + 3 * (size_t)blockIdx.x);
+}
+
+// WilsonLoop chain update (wilson_loop.hpp:71-88): chain(x, mu) <- [1 or chain(x, mu)]
+// * U_mu(x + n0 mu) ... U_mu(x + (n1 - 1) mu), one thread per (site, mu)
+__global__ void __launch_bounds__(BS) wchain_kernel(const double2* __restrict__ U, double2* __restrict__ chain, Geom g,
+                                                    int n0, int n1, int reset)
+{
+    const long long id = (long long)blockIdx.x * BS + threadIdx.x;
+    if (id >= 4LL * g.V) return;
+    const int s = (int)(id >> 2), mu = (int)(id & 3);
+    int c[4];
+    coords_of(g, s, c);
+    const int par = parity_of(c);
+    M3 a = reset ? m3_id() : ld(chain, g, s, mu, par);
+    for (int n = n0; n < n1; ++n) {
+        const int sn = shift(g, s, c, mu, n % g.L[mu]);
+        a = mul(a, link(U, g, sn, mu));
+    }
+    st_link(chain, g, s, mu, par, a);
+}
+
+// the square loops of side L from the chains (wilson_loop.hpp:89-125), summed over
+// the six planes: Re tr (chain_mu(x) chain_nu(x + L mu) (chain_nu(x) chain_mu(x + L nu))^+)
+__global__ void __launch_bounds__(BS) wloop_kernel(const double2* __restrict__ chain, Geom g, int L, double* part)
+{
+    const int s = blockIdx.x * BS + threadIdx.x;
+    double w = 0.0;
+    if (s < g.V) {
+        int c[4];
+        coords_of(g, s, c);
+        const int par = parity_of(c);
+        auto ch = [&](int site, int mu) {
+            int cc[4];
+            coords_of(g, site, cc);
+            return ld(chain, g, site, mu, parity_of(cc));
+        };
+        for (int nu = 1; nu < 4; ++nu)
+            for (int mu = 0; mu < nu; ++mu) {
+                const M3 A = mul(ld(chain, g, s, mu, par), ch(shift(g, s, c, mu, L % g.L[mu]), nu));
+                const M3 B = mul(ld(chain, g, s, nu, par), ch(shift(g, s, c, nu, L % g.L[nu]), mu));
+                w += retr_mul(A, dag(B));
+            }
+    }
+    block_sum2(w, 0.0, part + 2 * (size_t)blockIdx.x);
+}
+
+// PolyakovLoop (polyakov_loop.hpp:31-49): one thread per spatial site, tr prod_t U_t
+__global__ void __launch_bounds__(BS) polyakov_kernel(const double2* __restrict__ U, Geom g, double* part)
+{
+    const int vs = g.L[1] * g.L[2] * g.L[3];
+    const int r = blockIdx.x * BS + threadIdx.x;
+    double re = 0.0, im = 0.0;
+    if (r < vs) {
+        M3 a = m3_id();
+        for (int t = 0; t < g.L[0]; ++t) a = mul(a, link(U, g, t * vs + r, 0));
+        re = a.m[0].re + a.m[4].re + a.m[8].re;
+        im = a.m[0].im + a.m[4].im + a.m[8].im;
+    }
+    block_sum2(re, im, part + 2 * (size_t)blockIdx.x);
+}
+
 }  // namespace
+
+// sums over sites of Re tr P, Re tr (R12 + R21) and the raw plaquette charge density
+void k_rect_sums(lt_field u, double out[3])
+{
+    lt_ctx ctx = u->ctx;
+    const unsigned nb = 3 * nblocks(u->g.V, BS);
+    double* part = ctx->scratch_for(3 * (size_t)nb);
+    launch(ctx, "rect", rect_kernel, dim3(nb), dim3(BS), 0, (const double2*)u->d, u->g, part);
+    double* h = ctx->host_for(3 * (size_t)nb);
+    LT_HIP_CHECK(hipMemcpyAsync(h, part, 3 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    out[0] = out[1] = out[2] = 0.0;
+    for (unsigned i = 0; i < nb; ++i)
+        for (int k = 0; k < 3; ++k) out[k] += h[3 * i + k];
+}
+
+double k_wilson_loop(lt_field u, lt_field chain, int n0, int n1, int reset)
+{
+    lt_ctx ctx = u->ctx;
+    const Geom& g = u->g;
+    launch(ctx, "wilson_chain", wchain_kernel, dim3(nblocks(4LL * g.V, BS)), dim3(BS), 0, (const double2*)u->d,
+           chain->d, g, n0, n1, reset);
+    const unsigned nb = nblocks(g.V, BS);
+    double* part = ctx->scratch_for(2 * (size_t)nb);
+    launch(ctx, "wilson_loop", wloop_kernel, dim3(nb), dim3(BS), 0, (const double2*)chain->d, g, n1, part);
+    double* h = ctx->host_for(2 * (size_t)nb);
+    LT_HIP_CHECK(hipMemcpyAsync(h, part, 2 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double w = 0.0;
+    for (unsigned i = 0; i < nb; ++i) w += h[2 * i];
+    return w;
+}
+
+void k_polyakov_loop(lt_field u, double* re, double* im)
+{
+    lt_ctx ctx = u->ctx;
+    const Geom& g = u->g;
+    const int vs = g.L[1] * g.L[2] * g.L[3];
+    const unsigned nb = nblocks(vs, BS);
+    double* part = ctx->scratch_for(2 * (size_t)nb);
+    launch(ctx, "polyakov_loop", polyakov_kernel, dim3(nb), dim3(BS), 0, (const double2*)u->d, g, part);
+    double* h = ctx->host_for(2 * (size_t)nb);
+    LT_HIP_CHECK(hipMemcpyAsync(h, part, 2 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double a = 0.0, b = 0.0;
+    for (unsigned i = 0; i < nb; ++i) {
+        a += h[2 * i];
+        b += h[2 * i + 1];
+    }
+    *re = a / vs;
+    *im = b / vs;
+}
 
 void k_observables(lt_field u, double* host_out)
 {

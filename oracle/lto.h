@@ -46,6 +46,10 @@ void   lto_reunitarize(double *U, lto_dims d);
 void   lto_su3_exp(const double *Q, double *out);           /* exp(iQ), Q traceless hermitian */
 void   lto_gram_schmidt(double *M);
 void   lto_third_row_fma(double *U, size_t nlinks);
+void   lto_rect_sums(const double *U, lto_dims d, double *plaq_sum, double *rect_sum);
+double lto_topcharge_plaquette(const double *U, lto_dims d);
+double lto_wilson_loop(const double *U, double *chain, lto_dims d, int n0, int n1, int reset);
+void   lto_polyakov_loop(const double *U, lto_dims d, double *re, double *im);
 void   lto_algebra(const double *M, double *out);           /* traceless anti-hermitian part */
 
 /* ---- actions / updates --------------------------------------------------- */

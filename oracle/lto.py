@@ -67,6 +67,10 @@ def _load():
         "lto_field_update": (None, [D, D, dims, ctypes.c_double]),
         "lto_kinetic": (ctypes.c_double, [D, dims]),
         "lto_third_row_fma": (None, [D, ctypes.c_size_t]),
+        "lto_rect_sums": (None, [D, dims, D, D]),
+        "lto_topcharge_plaquette": (ctypes.c_double, [D, dims]),
+        "lto_wilson_loop": (ctypes.c_double, [D, D, dims, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "lto_polyakov_loop": (None, [D, dims, D, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -466,3 +470,37 @@ def bias_file_text(cv_min, cv_max, bins, weight, width, wt_param, threshold, gri
              "well_tempered_parameter: " + g(wt_param), "threshold_weight: " + g(threshold), "END_METADYN_PARAMS",
              ",".join(g(v) for v in grid), "exceeded_count: %d" % exceeded]
     return "\n".join(lines) + "\n"
+
+
+# ---------------------------------------------------------------------------
+# the rest of Observables() (SU3_main.cpp:89-136)
+
+def rect_sums(U, shape):
+    """sum Re tr P_munu and sum Re tr (R12 + R21) over x, mu < nu (rectangular_action.hpp:61-95)."""
+    p = ctypes.c_double()
+    r = ctypes.c_double()
+    lib().lto_rect_sums(_p(U), dims(shape), ctypes.byref(p), ctypes.byref(r))
+    return p.value, r.value
+
+
+def rect_action_normalized(U, shape, c_plaq, c_rect):
+    """Rectangular<2>::ActionNormalized = Action / (6 beta V) (rectangular_action.hpp:88-100)."""
+    V = volume(shape)
+    sp, sr = rect_sums(U, shape)
+    return (c_plaq * (6 * V - sp / 3.0) + c_rect * (12 * V - sr / 3.0)) / (6.0 * V)
+
+
+def topcharge_plaquette(U, shape):
+    return lib().lto_topcharge_plaquette(_p(U), dims(shape))
+
+
+def wilson_loop(U, chain, shape, n0, n1, reset):
+    """WilsonLoop<n0, n1, reset>(U, chain); chain (V, 4, 18) is updated in place."""
+    return lib().lto_wilson_loop(_p(U), _p(chain), dims(shape), n0, n1, 1 if reset else 0)
+
+
+def polyakov_loop(U, shape):
+    re = ctypes.c_double()
+    im = ctypes.c_double()
+    lib().lto_polyakov_loop(_p(U), dims(shape), ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)

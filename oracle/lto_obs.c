@@ -239,3 +239,105 @@ void lto_flow_rk3(double *U, double *Z, lto_dims d, double eps, int n_steps)
         flow_update_fields(U, Z, d, 1.0);
     }
 }
+
+/* ---- the rest of Observables() (SU3_main.cpp:89-136) ---------------------------- */
+
+/* 1x2 and 2x1 loops (wilson_loop.hpp:155-175): 2 links along mu (or nu) first */
+static mat3 rect12(const double *U, lto_dims d, size_t s, int mu, int nu)
+{
+    size_t smu = lto_move(d, s, mu, 1), smu_nu = lto_move(d, smu, nu, 1);
+    size_t snu = lto_move(d, s, nu, 1), snu2 = lto_move(d, snu, nu, 1);
+    mat3 a = m_mul(m_mul(lto_link(U, s, mu), lto_link(U, smu, nu)), lto_link(U, smu_nu, nu));
+    a = m_mul_nd(m_mul_nd(m_mul_nd(a, lto_link(U, snu2, mu)), lto_link(U, snu, nu)), lto_link(U, s, nu));
+    return a;
+}
+static mat3 rect21(const double *U, lto_dims d, size_t s, int mu, int nu)
+{
+    size_t smu = lto_move(d, s, mu, 1), smu2 = lto_move(d, smu, mu, 1);
+    size_t smu_nu = lto_move(d, smu, nu, 1), snu = lto_move(d, s, nu, 1);
+    mat3 a = m_mul(m_mul(lto_link(U, s, mu), lto_link(U, smu, mu)), lto_link(U, smu2, nu));
+    a = m_mul_nd(m_mul_nd(m_mul_nd(a, lto_link(U, smu_nu, mu)), lto_link(U, snu, mu)), lto_link(U, s, nu));
+    return a;
+}
+
+/* Rectangular<2>::Action (rectangular_action.hpp:61-95): sums over x and mu < nu of
+ * Re tr P_munu and Re tr (R12 + R21) */
+void lto_rect_sums(const double *U, lto_dims d, double *plaq_sum, double *rect_sum)
+{
+    size_t V = lto_volume(d);
+    double sp = 0.0, sr = 0.0;
+    #pragma omp parallel for reduction(+:sp, sr)
+    for (size_t s = 0; s < V; ++s)
+        for (int nu = 1; nu < 4; ++nu)
+            for (int mu = 0; mu < nu; ++mu) {
+                sp += creal(m_tr(plaq(U, d, s, mu, nu)));
+                sr += creal(m_tr(rect12(U, d, s, mu, nu))) + creal(m_tr(rect21(U, d, s, mu, nu)));
+            }
+    *plaq_sum = sp;
+    *rect_sum = sr;
+}
+
+/* TopChargePlaquette (topological_charge.hpp:366-416): F_munu = -i (P_munu - P_munu^+)
+ * with single plaquettes, Q = 1/(16 pi^2) sum_x Re tr (F01 F23 + F02 F31 + F03 F12) */
+double lto_topcharge_plaquette(const double *U, lto_dims d)
+{
+    size_t V = lto_volume(d);
+    double q = 0.0;
+    #pragma omp parallel for reduction(+:q)
+    for (size_t s = 0; s < V; ++s) {
+        mat3 F[6];
+        for (int p = 0; p < 6; ++p) {
+            mat3 P = plaq(U, d, s, PL_MU[p], PL_NU[p]);
+            F[p] = m_scale(-I, m_sub(P, m_dag(P)));
+        }
+        /* planes 01 02 03 12 13 23; F31 = -F13 */
+        q += m_retr_mul(F[0], F[5]) - m_retr_mul(F[1], F[4]) + m_retr_mul(F[2], F[3]);
+    }
+    return q / (16.0 * LTO_PI * LTO_PI);
+}
+
+/* WilsonLoop<n0, n1, reset> (wilson_loop.hpp:65-126): chain(x, mu) is multiplied by the
+ * links U_mu(x + n mu) for n0 <= n < n1 (after being reset to 1 when asked); then the
+ * square loops of side n1 built from the chains, summed over x and the six planes:
+ * returns 1 - sum Re tr / (18 V).  chain: V * 4 general matrices, kept between calls. */
+double lto_wilson_loop(const double *U, double *chain, lto_dims d, int n0, int n1, int reset)
+{
+    size_t V = lto_volume(d);
+    #pragma omp parallel for
+    for (size_t s = 0; s < V; ++s)
+        for (int mu = 0; mu < 4; ++mu) {
+            mat3 c = reset ? m_id() : lto_link(chain, s, mu);
+            for (int n = n0; n < n1; ++n) c = m_mul(c, lto_link(U, lto_move(d, s, mu, n), mu));
+            lto_set_link(chain, s, mu, c);
+        }
+    double w = 0.0;
+    #pragma omp parallel for reduction(+:w)
+    for (size_t s = 0; s < V; ++s)
+        for (int p = 0; p < 6; ++p) {
+            int mu = PL_MU[p], nu = PL_NU[p];
+            mat3 a = m_mul(lto_link(chain, s, mu), lto_link(chain, lto_move(d, s, mu, n1), nu));
+            a = m_mul_nd(m_mul_nd(a, lto_link(chain, lto_move(d, s, nu, n1), mu)), lto_link(chain, s, nu));
+            w += creal(m_tr(a));
+        }
+    return 1.0 - w / (18.0 * (double)V);
+}
+
+/* PolyakovLoop (polyakov_loop.hpp:31-49): (1/V_s) sum over spatial sites of
+ * tr prod_t U_t(t, x) */
+void lto_polyakov_loop(const double *U, lto_dims d, double *re, double *im)
+{
+    double sr = 0.0, si = 0.0;
+    #pragma omp parallel for collapse(3) reduction(+:sr, si)
+    for (int x = 0; x < d.nx; ++x)
+        for (int y = 0; y < d.ny; ++y)
+            for (int z = 0; z < d.nz; ++z) {
+                mat3 c = m_id();
+                for (int t = 0; t < d.nt; ++t) c = m_mul(c, lto_link(U, lto_site_index(d, t, x, y, z), 0));
+                cplx tr = m_tr(c);
+                sr += creal(tr);
+                si += cimag(tr);
+            }
+    double vs = (double)d.nx * d.ny * d.nz;
+    *re = sr / vs;
+    *im = si / vs;
+}

@@ -30,6 +30,8 @@
 #include "LettuceGauge/updates/hmc_metadynamics.hpp"
 #include "LettuceGauge/updates/tempering.hpp"
 #include "LettuceGauge/IO/config_io/bmw_format.hpp"
+#include "LettuceGauge/observables/wilson_loop.hpp"
+#include "LettuceGauge/observables/polyakov_loop.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -411,6 +413,20 @@ int main(int argc, char** argv)
         GradientFlowKernel Flow(*U, *U2, I, GaugeAction::WilsonAction, 0.01);
         double tfl = time_it([&] { Flow(nrep); }) / nrep;
         std::printf("hb_sweep_s %.6g\nor_sweep_s %.6g\nrk3_step_s %.6g\nthreads %d\n", thb, tor, tfl, omp_get_max_threads());
+    } else if (cmd == "obs2") {                  // obs2 in out beta: the rest of Observables() (SU3_main.cpp:125-136)
+        read_field(argv[2], *U);
+        const double beta = std::atof(argv[4]);
+        GaugeAction::Rectangular<2> Symanzik(beta, 1.0 + 8.0 * 1.0 / 12.0, -1.0 / 12.0);
+        std::vector<double> out;
+        out.push_back(Symanzik.ActionNormalized(*U));
+        out.push_back(TopChargePlaquette(*U));
+        out.push_back(WilsonLoop<0, 2, true>(*U, *U2));
+        out.push_back(WilsonLoop<2, 4, false>(*U, *U2));
+        out.push_back(WilsonLoop<4, 8, false>(*U, *U2));
+        const std::complex<double> P = PolyakovLoop(*U);
+        out.push_back(std::real(P));
+        out.push_back(std::imag(P));
+        write_values(argv[3], out);
     } else if (cmd == "bmw_save") {              // bmw_save in out full(0|1): the reference's BMW writer
         read_field(argv[2], *U);
         const bool ok = std::atoi(argv[4]) ? SaveConfigBMWFull(*U, argv[3], true) : SaveConfigBMW(*U, argv[3], true);
