@@ -325,6 +325,20 @@ def main():
     for _ in range(10):
         la.Observables(W)
     obs_ms = 100.0 * (time.perf_counter() - t0)
+    # the rest of the reference's Observables(): Symanzik action, plaquette charge,
+    # Wilson loops L = 2, 4, 8 on one chain workspace, Polyakov loop
+    chain = la.MatrixField(ctx, shape)
+    sym = la.RectangularAction(args.beta, 1.0 + 8.0 / 12.0, -1.0 / 12.0)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        sym.ActionNormalized(W)
+        la.TopChargePlaquette(W)
+        la.WilsonLoop(0, 2, True, W, chain)
+        la.WilsonLoop(2, 4, False, W, chain)
+        la.WilsonLoop(4, 8, False, W, chain)
+        la.PolyakovLoop(W)
+    obs_rest_ms = (time.perf_counter() - t0) / 3 * 1e3
+    chain.close()
     fl.close()
     W.close()
 
@@ -358,6 +372,7 @@ def main():
                      "stage_GBs": BYTES_PER_LINK_STAGE * 4 * V / (stage_ms * 1e-3) / 1e9,
                      "link_stage_updates_per_s": 12 * V / (flow_ms * 1e-3)},
         "observables_pass_ms": obs_ms,
+        "observables_rest_ms": obs_rest_ms,
         "plaquette": plaq,
         "thermalisation_s": t_therm,
     }
