@@ -120,6 +120,10 @@ int lt_staple_field(lt_field u, lt_field out);
 /* n sweeps of OverrelaxationSubgroupKernel driven by Iterator::Checkerboard[4]
  * (overrelaxation.hpp:68-117, iterators.hpp:65-169) */
 int lt_or_sweep(lt_field u, int n_sweep, int schedule);
+/* ParityUpdate (updates/parity_update.hpp:22-47): the spatial reflection x -> N-1-x,
+ * which keeps the action and maps the topological charge Q to -Q; u_copy (another SU(3)
+ * field of the same extents) receives the old links */
+int lt_parity_update(lt_field u, lt_field u_copy);
 /* n sweeps of HeatbathKernel (heatbath.hpp:30-130) at coupling beta; draws come
  * from the field's Philox state, whose counter advances by one per sweep */
 int lt_heatbath_sweep(lt_field u, double beta, int n_sweep, int schedule);

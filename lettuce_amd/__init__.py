@@ -34,6 +34,7 @@ __all__ = [
     "StoutSmearing4D", "StoutSmearingN", "GradientFlowKernel", "Observables", "LettuceError",
     "CB2", "CB4", "SaveConfigBMW", "SaveConfigBMWFull", "LoadConfigBMW", "LoadConfigBMWFull", "CheckpointManager",
     "PhiloxState", "RectangularAction", "TopChargePlaquette", "WilsonLoop", "PolyakovLoop",
+    "ParityUpdate",
 ]
 
 CB2 = 2
@@ -312,6 +313,11 @@ class RectangularAction:
 
     def Action(self, U):
         return self.ActionNormalized(U) * 6.0 * self.beta * U.volume
+
+
+def ParityUpdate(U, U_copy):
+    """ParityUpdate (updates/parity_update.hpp:22-47): spatial reflection, Q -> -Q."""
+    call("lt_parity_update", U.handle, U_copy.handle)
 
 
 def TopChargePlaquette(U):

@@ -544,6 +544,21 @@ int lt_observables(lt_field u, double* out)
     });
 }
 
+int lt_parity_update(lt_field u, lt_field u_copy)
+{
+    if (!u || !u_copy) return fail(LT_EINVAL, "null argument");
+    if (u == u_copy) return fail(LT_EINVAL, "lt_parity_update: the copy must be another field");
+    if (!u->su3 || !u_copy->su3) return fail(LT_ESTATE, "lt_parity_update: SU(3) fields required");
+    if (!same_shape(u, u_copy) || u->ctx != u_copy->ctx) return fail(LT_ESTATE, "lt_parity_update: shape/context mismatch");
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_copy(u->ctx, u->d, u_copy->d, (size_t)72 * u->g.Vh);
+        lt::k_parity(u, u_copy->d);
+        LT_HIP_CHECK(hipStreamSynchronize(u->ctx->stream));
+        return LT_OK;
+    });
+}
+
 int lt_action_rectangular(lt_field u, double c_plaq, double c_rect, double* action_normalized)
 {
     if (!u || !action_normalized) return fail(LT_EINVAL, "null argument");

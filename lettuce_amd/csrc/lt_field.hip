@@ -126,6 +126,23 @@ __global__ void __launch_bounds__(BS) check_kernel(const double2* __restrict__ U
     }
 }
 
+// ParityUpdate (updates/parity_update.hpp:22-47): with P x = (t, Nx-1-x, Ny-1-y, Nz-1-z),
+// U'_t(x) = U_t(P x) and U'_i(x) = U_i(P x - i)^+ for the spatial directions; one thread
+// per (site, mu), reading the copy `src` and writing `dst`
+__global__ void __launch_bounds__(BS) parity_kernel(const double2* __restrict__ src, double2* __restrict__ dst, Geom g)
+{
+    const long long id = (long long)blockIdx.x * BS + threadIdx.x;
+    if (id >= 4LL * g.V) return;
+    const int s = (int)(id >> 2), mu = (int)(id & 3);
+    int c[4];
+    coords_of(g, s, c);
+    int r[4] = {c[0], g.L[1] - 1 - c[1], g.L[2] - 1 - c[2], g.L[3] - 1 - c[3]};
+    if (mu > 0) r[mu] = r[mu] == 0 ? g.L[mu] - 1 : r[mu] - 1;
+    const int sr = ((r[0] * g.L[1] + r[1]) * g.L[2] + r[2]) * g.L[3] + r[3];
+    const M3 a = ldu(src, g, sr, mu, parity_of(r));
+    stu(dst, g, s, mu, parity_of(c), mu == 0 ? a : dag(a));
+}
+
 // plain streaming copy of a link buffer (16-byte elements)
 __global__ void __launch_bounds__(BS) copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst, size_t n)
 {
@@ -152,6 +169,11 @@ void k_unpack(lt_field f, double* d_aos)
 {
     launch(f->ctx, "unpack", unpack_kernel, dim3(nblocks(4LL * f->g.V, BS)), dim3(BS), 0, (const double2*)f->d, d_aos,
            f->g, (int)f->su3);
+}
+
+void k_parity(lt_field f, const double2* copy)
+{
+    launch(f->ctx, "parity_update", parity_kernel, dim3(nblocks(4LL * f->g.V, BS)), dim3(BS), 0, copy, f->d, f->g);
 }
 
 void k_identity(lt_field f)

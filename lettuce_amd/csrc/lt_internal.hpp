@@ -215,6 +215,7 @@ void k_hb_sweep(lt_field u, double beta, int n, int schedule);
 void k_bmw_pack(lt_field f, int K, long long n0, long long count, uint64_t* d_out, uint64_t* d_part);
 void k_bmw_unpack(lt_field f, int K, long long n0, long long count, const uint64_t* d_in, uint64_t* d_part,
                   double* d_dev);
+void k_parity(lt_field f, const double2* copy);
 void k_observables(lt_field u, double* host_out);
 void k_rect_sums(lt_field u, double out[3]);
 double k_wilson_loop(lt_field u, lt_field chain, int n0, int n1, int reset);

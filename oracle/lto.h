@@ -50,6 +50,7 @@ void   lto_rect_sums(const double *U, lto_dims d, double *plaq_sum, double *rect
 double lto_topcharge_plaquette(const double *U, lto_dims d);
 double lto_wilson_loop(const double *U, double *chain, lto_dims d, int n0, int n1, int reset);
 void   lto_polyakov_loop(const double *U, lto_dims d, double *re, double *im);
+void   lto_parity_update(const double *U, double *out, lto_dims d);
 void   lto_algebra(const double *M, double *out);           /* traceless anti-hermitian part */
 
 /* ---- actions / updates --------------------------------------------------- */

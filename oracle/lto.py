@@ -71,6 +71,7 @@ def _load():
         "lto_topcharge_plaquette": (ctypes.c_double, [D, dims]),
         "lto_wilson_loop": (ctypes.c_double, [D, D, dims, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "lto_polyakov_loop": (None, [D, dims, D, D]),
+        "lto_parity_update": (None, [D, D, dims]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -504,3 +505,10 @@ def polyakov_loop(U, shape):
     im = ctypes.c_double()
     lib().lto_polyakov_loop(_p(U), dims(shape), ctypes.byref(re), ctypes.byref(im))
     return complex(re.value, im.value)
+
+
+def parity_update(U, shape):
+    """ParityUpdate (updates/parity_update.hpp:22-47): a new field."""
+    out = new_field(shape)
+    lib().lto_parity_update(_p(U), _p(out), dims(shape))
+    return out

@@ -341,3 +341,21 @@ void lto_polyakov_loop(const double *U, lto_dims d, double *re, double *im)
     *re = sr / vs;
     *im = si / vs;
 }
+
+/* ParityUpdate (updates/parity_update.hpp:22-47): with P x = (t, Nx-1-x, Ny-1-y, Nz-1-z),
+ * out_t(x) = U_t(P x), out_i(x) = U_i(P x - i)^+ (i spatial) */
+void lto_parity_update(const double *U, double *out, lto_dims d)
+{
+    size_t V = lto_volume(d);
+    #pragma omp parallel for
+    for (size_t s = 0; s < V; ++s) {
+        lto_site c = lto_site_coord(d, s);
+        int r[4] = {c.c[0], d.nx - 1 - c.c[1], d.ny - 1 - c.c[2], d.nz - 1 - c.c[3]};
+        for (int mu = 0; mu < 4; ++mu) {
+            int q[4] = {r[0], r[1], r[2], r[3]};
+            if (mu > 0) q[mu] = (q[mu] - 1 + lto_len(d, mu)) % lto_len(d, mu);
+            mat3 a = lto_link(U, lto_site_index(d, q[0], q[1], q[2], q[3]), mu);
+            lto_set_link(out, s, mu, mu == 0 ? a : m_dag(a));
+        }
+    }
+}

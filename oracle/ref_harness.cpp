@@ -32,6 +32,7 @@
 #include "LettuceGauge/IO/config_io/bmw_format.hpp"
 #include "LettuceGauge/observables/wilson_loop.hpp"
 #include "LettuceGauge/observables/polyakov_loop.hpp"
+#include "LettuceGauge/updates/parity_update.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -427,6 +428,10 @@ int main(int argc, char** argv)
         out.push_back(std::real(P));
         out.push_back(std::imag(P));
         write_values(argv[3], out);
+    } else if (cmd == "parity") {                // parity in out: ParityUpdate (parity_update.hpp:22-47)
+        read_field(argv[2], *U);
+        ParityUpdate(*U, *U2);
+        write_field(argv[3], *U);
     } else if (cmd == "bmw_save") {              // bmw_save in out full(0|1): the reference's BMW writer
         read_field(argv[2], *U);
         const bool ok = std::atoi(argv[4]) ? SaveConfigBMWFull(*U, argv[3], true) : SaveConfigBMW(*U, argv[3], true);

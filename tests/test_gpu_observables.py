@@ -72,3 +72,25 @@ def test_unit_field(la, lto, gpu_ctx):
     shape = (4, 4, 6, 8)
     got = gpu_values(la, gpu_ctx, lto.identity(shape), shape)
     assert np.array_equal(got, np.array([0.0, 0.0, 0.0, 0.0, 0.0, 3.0, 0.0]))
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16, 16), (4, 6, 8, 10)])
+def test_parity_update(la, lto, gpu_ctx, shape):
+    """ParityUpdate on the GPU against the oracle (the oracle matches the reference at
+    16^4), the action kept, the clover charge reversed, applied twice the identity."""
+    flush(f"parity update {shape}")
+    U = lto.stout(lto.haar(shape, 0), shape, 0.1, 3)
+    f = la.GaugeField(gpu_ctx, shape)
+    f.upload(U)
+    tmp = la.GaugeField(gpu_ctx, shape)
+    q0 = la.TopChargeClover(f)
+    p0 = la.PlaquetteSum(f)
+    la.ParityUpdate(f, tmp)
+    got = f.download()
+    ref = lto.parity_update(U, shape)
+    assert np.max(np.abs(got - ref)) < 1e-14
+    assert abs(la.TopChargeClover(f) + q0) < 1e-12 and abs(la.PlaquetteSum(f) - p0) < 1e-9
+    la.ParityUpdate(f, tmp)
+    assert np.max(np.abs(f.download() - U)) < 1e-14
+    f.close()
+    tmp.close()
