@@ -229,6 +229,14 @@ int lt_bias_derivative(lt_bias b, double cv, double *d);
 int lt_bias_grid(lt_bias b, double *values, int n, int set);
 int lt_bias_cv(lt_bias b, double *cv, int set);
 
+/* MetadynamicsTemperingKernel::operator() (updates/tempering.hpp:57-72): two streams, U
+ * (unbiased local updates) and U_temper (MetaD-HMC with `bias`).  CV_new = the clover charge
+ * of U after the CV's stout levels, dV = V(CV_new) - V(bias CV_current); with q the Philox
+ * uniform of stream 4, site 0, key seed, counter (the reference draws prng.UniformReal()),
+ * q <= exp(-dV) swaps the links of U and U_temper and sets the bias CV to CV_new */
+int lt_metad_tempering_swap(lt_field U, lt_field U_temper, lt_cv cv, lt_bias bias, uint64_t seed, uint64_t counter,
+                            int *accepted, double *delta_v);
+
 /* MetaBiasPotential::SaveParameters / SavePotential / LoadPotential
  * (metadynamics.hpp:304-505): the reference's text format (version line, parameter
  * lines up to END_METADYN_PARAMS, comma-separated grid, exceeded_count); overwrite = 0

@@ -34,7 +34,7 @@ __all__ = [
     "StoutSmearing4D", "StoutSmearingN", "GradientFlowKernel", "Observables", "LettuceError",
     "CB2", "CB4", "SaveConfigBMW", "SaveConfigBMWFull", "LoadConfigBMW", "LoadConfigBMWFull", "CheckpointManager",
     "PhiloxState", "RectangularAction", "TopChargePlaquette", "WilsonLoop", "PolyakovLoop",
-    "ParityUpdate",
+    "ParityUpdate", "MetadynamicsTemperingKernel",
 ]
 
 CB2 = 2
@@ -605,6 +605,28 @@ class HMCMetaDKernel:
         self.delta_H = d.value
         self.cv_value = q.value
         return bool(a.value)
+
+
+class MetadynamicsTemperingKernel:
+    """MetadynamicsTemperingKernel (updates/tempering.hpp:24-73): swap proposals between
+    the unbiased stream U and the MetaD stream U_temper; each call draws from Philox
+    stream 4 at (seed, counter) and advances the counter."""
+
+    def __init__(self, U, U_temper, bias, cv, seed=0, counter=0):
+        self.U, self.U_temper, self.bias, self.cv = U, U_temper, bias, cv
+        self.seed, self.counter = seed, counter
+        self.delta_V = 0.0
+        self.acceptances = 0
+
+    def __call__(self):
+        acc = ctypes.c_int()
+        dv = ctypes.c_double()
+        call("lt_metad_tempering_swap", self.U.handle, self.U_temper.handle, self.cv.handle, self.bias.handle,
+             self.seed, self.counter, ctypes.byref(acc), ctypes.byref(dv))
+        self.counter += 1
+        self.delta_V = dv.value
+        self.acceptances += acc.value
+        return bool(acc.value)
 
 
 class _ReplicaField(GaugeField):

@@ -517,7 +517,12 @@ __device__ __forceinline__ M3 exp_ah(const M3& A, double c)
 //   key = seed, ctr = {site, stream<<24 | k<<4 | sub<<2 | mu, counter_lo, counter_hi}
 enum : uint32_t { RNG_HEATBATH = 0, RNG_MOMENTUM = 1, RNG_HAAR = 2 };
 
-__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1)
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+}
+
+__host__ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -525,9 +530,9 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t hi0 = mulhi32(0xD2511F53u, c[0]);
         uint32_t lo0 = 0xD2511F53u * c[0];
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t hi1 = mulhi32(0xCD9E8D57u, c[2]);
         uint32_t lo1 = 0xCD9E8D57u * c[2];
         uint32_t n0 = hi1 ^ c[1] ^ k0;
         uint32_t n2 = hi0 ^ c[3] ^ k1;
@@ -538,13 +543,13 @@ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1
     }
 }
 
-__device__ __forceinline__ double u01(uint32_t a, uint32_t b)
+__host__ __device__ __forceinline__ double u01(uint32_t a, uint32_t b)
 {
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
-__device__ __forceinline__ void draw2(uint64_t seed, uint32_t site, uint32_t stream, uint32_t k, uint32_t sub,
-                                      uint32_t mu, uint64_t counter, double& u0, double& u1)
+__host__ __device__ __forceinline__ void draw2(uint64_t seed, uint32_t site, uint32_t stream, uint32_t k,
+                                               uint32_t sub, uint32_t mu, uint64_t counter, double& u0, double& u1)
 {
     uint32_t c[4] = {site, (stream << 24) | (k << 4) | (sub << 2) | mu, (uint32_t)counter, (uint32_t)(counter >> 32)};
     philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));

@@ -425,6 +425,35 @@ int lt_cv_forward(lt_cv cv, lt_field U, double* Q)
     });
 }
 
+// MetadynamicsTemperingKernel::operator() (tempering.hpp:57-72): the unbiased stream's
+// smeared clover charge against the bias's current CV (the biased stream's), a Philox
+// uniform from stream 4 at site 0 (the draw of label 0 in a replica ladder), and a
+// swap of the two fields' links on acceptance.
+int lt_metad_tempering_swap(lt_field U, lt_field U_temper, lt_cv cv, lt_bias bias, uint64_t seed, uint64_t counter,
+                            int* accepted, double* delta_v)
+{
+    if (!U || !U_temper || !cv || !bias) return fail(LT_EINVAL, "lt_metad_tempering_swap: null argument");
+    if (!same_shape(U->g, U_temper->g) || !same_shape(cv->g, U->g) || U->ctx != U_temper->ctx || cv->ctx != U->ctx)
+        return fail(LT_EINVAL, "lt_metad_tempering_swap: shape/context mismatch");
+    if (!U->su3 || !U_temper->su3) return fail(LT_ESTATE, "lt_metad_tempering_swap: SU(3) fields required");
+    if (U->view || U_temper->view) return fail(LT_ESTATE, "lt_metad_tempering_swap: replicas of a batch cannot be swapped");
+    return guarded([&] {
+        const double cv_old = bias->cv_current;
+        const double cv_new = cv_forward(cv, U);
+        const double dv = bias->potential(cv_new) - bias->potential(cv_old);
+        double u0 = 0.0, u1 = 0.0;
+        lt::draw2(seed, 0u, 4u, 0u, 0u, 0u, counter, u0, u1);
+        const bool acc = u0 <= std::exp(-dv);
+        if (acc) {
+            std::swap(U->d, U_temper->d);
+            bias->cv_current = cv_new;
+        }
+        if (accepted) *accepted = acc ? 1 : 0;
+        if (delta_v) *delta_v = dv;
+        return LT_OK;
+    });
+}
+
 int lt_cv_backward(lt_cv cv, double dvdq, lt_field F)
 {
     if (!cv || !F || !same_shape(cv->g, F->g) || cv->ctx != F->ctx) return fail(LT_EINVAL, "lt_cv_backward: bad arguments");

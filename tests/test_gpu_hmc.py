@@ -294,3 +294,46 @@ def test_checkpointed_metad_trajectory_equals_full(la, lto, golden, gpu_ctx):
     (d0, a0, q0, u0, g0), (d1, a1, q1, u1, g1) = out
     assert d0 == d1 and a0 == a1 and q0 == q1
     assert np.array_equal(u0, u1) and np.array_equal(g0, g1)
+
+
+def test_two_stream_tempering_swap(la, lto, gpu_ctx):
+    """MetadynamicsTemperingKernel (tempering.hpp:57-72) against the oracle's rule, call by
+    call: CV of the unbiased stream through 4 stout levels, the bias at that CV and at the
+    biased stream's current CV, the Philox uniform of stream 4 / site 0; accepted calls
+    swap the links and move the bias CV."""
+    flush("two-stream tempering")
+    shape = (4, 4, 4, 8)
+    rho, nl = 0.12, 4
+    A = lto.stout(lto.haar(shape, 21), shape, 0.1, 2)
+    B = lto.stout(lto.haar(shape, 22), shape, 0.1, 2)
+    U = la.GaugeField(gpu_ctx, shape)
+    U.upload(A)
+    Ut = la.GaugeField(gpu_ctx, shape)
+    Ut.upload(B)
+    bias = la.MetaBiasPotential(-8.0, 8.0, 800, 1.5, 0.3, 10.0, 1000.0, well_tempered=True)
+    ob = lto.Bias(-8.0, 8.0, 800, 1.5, 0.3, 10.0, 1000.0, well_tempered=True)
+    for d in (0.3, -0.5, 1.1, 0.05):
+        bias.UpdatePotential(d)
+        ob.deposit(d)
+    cv = la.CollectiveVariable(gpu_ctx, shape, nl, rho)
+    cur = lto.cv_charge(B, shape, rho, nl)
+    bias.SetCV_current(cur)
+    kernel = la.MetadynamicsTemperingKernel(U, Ut, bias, cv, seed=77, counter=0)
+    host = {"U": A, "Ut": B}
+    seen = set()
+    for step in range(12):
+        q_new = lto.cv_charge(host["U"], shape, rho, nl)
+        dv = ob.potential(q_new) - ob.potential(cur)
+        expect = lto.uniform(77, 0, 4, step) <= np.exp(-dv)
+        got = kernel()
+        assert got == expect and abs(kernel.delta_V - dv) <= 1e-10 * max(1.0, abs(dv))
+        seen.add(got)
+        if got:
+            host["U"], host["Ut"] = host["Ut"], host["U"]
+            cur = q_new
+        assert abs(bias.ReturnCV_current() - cur) <= 1e-12
+        assert np.array_equal(U.download()[..., :12], host["U"][..., :12])
+        assert np.array_equal(Ut.download()[..., :12], host["Ut"][..., :12])
+    assert seen == {True, False}
+    for f in (U, Ut, cv, bias):
+        f.close()
