@@ -120,6 +120,14 @@ int lt_staple_field(lt_field u, lt_field out);
 /* n sweeps of OverrelaxationSubgroupKernel driven by Iterator::Checkerboard[4]
  * (overrelaxation.hpp:68-117, iterators.hpp:65-169) */
 int lt_or_sweep(lt_field u, int n_sweep, int schedule);
+/* Improved gauge actions Rectangular<2>(beta, c_plaq, c_rect) (rectangular_action.hpp:
+ * 29-300; Luescher-Weisz c_rect = -1/12, Iwasaki -0.331, DBW2 -1.4088, c_plaq = 1 - 8 c_rect):
+ * the staple c_plaq StaplePlaq + c_rect StapleRect into a matrix field, and overrelaxation /
+ * heatbath sweeps with it.  c_rect != 0 needs schedule LT_CB4 (Checkerboard4, extents
+ * divisible by 4); (1, 0) is the Wilson action of lt_or_sweep / lt_heatbath_sweep. */
+int lt_staple_field_action(lt_field u, double c_plaq, double c_rect, lt_field out);
+int lt_or_sweep_action(lt_field u, double c_plaq, double c_rect, int n_sweep, int schedule);
+int lt_heatbath_sweep_action(lt_field u, double beta, double c_plaq, double c_rect, int n_sweep, int schedule);
 /* ParityUpdate (updates/parity_update.hpp:22-47): the spatial reflection x -> N-1-x,
  * which keeps the action and maps the topological charge Q to -Q; u_copy (another SU(3)
  * field of the same extents) receives the old links */

@@ -34,7 +34,8 @@ __all__ = [
     "StoutSmearing4D", "StoutSmearingN", "GradientFlowKernel", "Observables", "LettuceError",
     "CB2", "CB4", "SaveConfigBMW", "SaveConfigBMWFull", "LoadConfigBMW", "LoadConfigBMWFull", "CheckpointManager",
     "PhiloxState", "RectangularAction", "TopChargePlaquette", "WilsonLoop", "PolyakovLoop",
-    "ParityUpdate", "MetadynamicsTemperingKernel",
+    "ParityUpdate", "MetadynamicsTemperingKernel", "LuescherWeiszAction", "IwasakiAction", "DBW2Action",
+    "StapleAction",
 ]
 
 CB2 = 2
@@ -221,22 +222,32 @@ class HeatbathKernel:
     """HeatbathKernel(U, Action, prng) (heatbath.hpp:106-108); beta plays the Action/global role,
     the field's Philox state plays the PRNG role."""
 
-    def __init__(self, U, beta):
+    def __init__(self, U, beta, action=None):
         self.U = U
         self.beta = float(beta)
+        self.action = action
 
     def sweep(self, n, schedule):
-        call("lt_heatbath_sweep", self.U.handle, self.beta, n, schedule)
+        if self.action is None:
+            call("lt_heatbath_sweep", self.U.handle, self.beta, n, schedule)
+        else:
+            call("lt_heatbath_sweep_action", self.U.handle, self.beta, self.action.c_plaq, self.action.c_rect, n,
+                 schedule)
 
 
 class OverrelaxationSubgroupKernel:
-    """OverrelaxationSubgroupKernel(U, Action) (overrelaxation.hpp:68-117)."""
+    """OverrelaxationSubgroupKernel(U, Action) (overrelaxation.hpp:68-117); action: a
+    RectangularAction (default: the Wilson action)."""
 
-    def __init__(self, U, beta=None):
+    def __init__(self, U, beta=None, action=None):
         self.U = U
+        self.action = action
 
     def sweep(self, n, schedule):
-        call("lt_or_sweep", self.U.handle, n, schedule)
+        if self.action is None:
+            call("lt_or_sweep", self.U.handle, n, schedule)
+        else:
+            call("lt_or_sweep_action", self.U.handle, self.action.c_plaq, self.action.c_rect, n, schedule)
 
 
 class Iterator:
@@ -318,6 +329,26 @@ class RectangularAction:
 def ParityUpdate(U, U_copy):
     """ParityUpdate (updates/parity_update.hpp:22-47): spatial reflection, Q -> -Q."""
     call("lt_parity_update", U.handle, U_copy.handle)
+
+
+def LuescherWeiszAction(beta):
+    """Rectangular<2>(beta, 1 + 8/12, -1/12) (rectangular_action.hpp:294)."""
+    return RectangularAction(beta, 1.0 + 8.0 / 12.0, -1.0 / 12.0)
+
+
+def IwasakiAction(beta):
+    """Rectangular<2>(beta, 1 + 8 * 0.331, -0.331) (rectangular_action.hpp:295)."""
+    return RectangularAction(beta, 1.0 + 8.0 * 0.331, -0.331)
+
+
+def DBW2Action(beta):
+    """Rectangular<2>(beta, 1 + 8 * 1.4088, -1.4088) (rectangular_action.hpp:296)."""
+    return RectangularAction(beta, 1.0 + 8.0 * 1.4088, -1.4088)
+
+
+def StapleAction(U, out, action):
+    """Rectangular<R>::Staple for every link into the MatrixField out."""
+    call("lt_staple_field_action", U.handle, action.c_plaq, action.c_rect, out.handle)
 
 
 def TopChargePlaquette(U):

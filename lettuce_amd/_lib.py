@@ -93,6 +93,9 @@ SIGNATURES = {
     "lt_batch_swap_stats": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), I],
     "lt_autotune_field": [c_field, I, ctypes.c_char_p, ctypes.POINTER(I), D],
     "lt_parity_update": [c_field, c_field],
+    "lt_staple_field_action": [c_field, ctypes.c_double, ctypes.c_double, c_field],
+    "lt_or_sweep_action": [c_field, ctypes.c_double, ctypes.c_double, I, I],
+    "lt_heatbath_sweep_action": [c_field, ctypes.c_double, ctypes.c_double, ctypes.c_double, I, I],
     "lt_metad_tempering_swap": [c_field, c_field, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                 ctypes.POINTER(I), D],
     "lt_action_rectangular": [c_field, ctypes.c_double, ctypes.c_double, D],

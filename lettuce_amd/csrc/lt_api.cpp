@@ -515,6 +515,59 @@ int lt_heatbath_sweep(lt_field u, double beta, int n_sweep, int schedule)
     });
 }
 
+// Rectangular<2>(beta, c_plaq, c_rect): the rectangle staples reach two links along mu,
+// so only the four-colour schedule keeps simultaneously updated links out of each
+// other's staples (Checkerboard4, iterators.hpp:141-169)
+static int check_action(lt_field u, int schedule, double c_plaq, double c_rect)
+{
+    if (!std::isfinite(c_plaq) || !std::isfinite(c_rect)) return fail(LT_EINVAL, "non-finite action coefficients");
+    if (!u->su3) return fail(LT_ESTATE, "the field must hold SU(3) links");
+    if (c_rect != 0.0 && schedule != LT_CB4) return fail(LT_EINVAL, "rectangle terms need the LT_CB4 schedule");
+    return check_schedule(u, schedule);
+}
+
+int lt_staple_field_action(lt_field u, double c_plaq, double c_rect, lt_field out)
+{
+    if (!u || !out) return fail(LT_EINVAL, "null field");
+    if (!same_shape(u, out) || u == out || u->ctx != out->ctx)
+        return fail(LT_ESTATE, "lt_staple_field_action: need a distinct field of equal shape");
+    if (!std::isfinite(c_plaq) || !std::isfinite(c_rect)) return fail(LT_EINVAL, "non-finite action coefficients");
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_staple_field_action(u, out, c_plaq, c_rect);
+        out->su3 = false;
+        return LT_OK;
+    });
+}
+
+int lt_or_sweep_action(lt_field u, double c_plaq, double c_rect, int n_sweep, int schedule)
+{
+    if (!u || n_sweep < 0) return fail(LT_EINVAL, "lt_or_sweep_action: bad argument");
+    int rc = check_action(u, schedule, c_plaq, c_rect);
+    if (rc) return rc;
+    if (c_rect == 0.0 && c_plaq == 1.0) return lt_or_sweep(u, n_sweep, schedule);
+    if (schedule != LT_CB4) return fail(LT_EINVAL, "lt_or_sweep_action: scaled actions use LT_CB4");
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_or_sweep_action(u, n_sweep, c_plaq, c_rect);
+        return LT_OK;
+    });
+}
+
+int lt_heatbath_sweep_action(lt_field u, double beta, double c_plaq, double c_rect, int n_sweep, int schedule)
+{
+    if (!u || n_sweep < 0 || !(beta > 0.0)) return fail(LT_EINVAL, "lt_heatbath_sweep_action: bad argument");
+    int rc = check_action(u, schedule, c_plaq, c_rect);
+    if (rc) return rc;
+    if (c_rect == 0.0 && c_plaq == 1.0) return lt_heatbath_sweep(u, beta, n_sweep, schedule);
+    if (schedule != LT_CB4) return fail(LT_EINVAL, "lt_heatbath_sweep_action: scaled actions use LT_CB4");
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_heatbath_sweep_action(u, beta, n_sweep, c_plaq, c_rect);
+        return LT_OK;
+    });
+}
+
 // out: [plaq_sum, Q, E_plaq, E_clov, q_t[Nt], e_t[Nt]]
 int lt_observables(lt_field u, double* out)
 {

@@ -1083,6 +1083,72 @@ __device__ __forceinline__ M3 staple_pipelined(const double2* __restrict__ U, co
     return st;
 }
 
+// site x + a e_mu + b e_nu (|a|, |b| <= 2, mu != nu) of site s with coordinates c
+__device__ __forceinline__ int wrap_add(int v, int a, int L)
+{
+    v += a;
+    return v < 0 ? v + L : (v >= L ? v - L : v);
+}
+__device__ __forceinline__ int shift2(const Geom& g, int s, const int c[4], int mu, int a, int nu, int b)
+{
+    const int cm = pick4(c, mu), cn = pick4(c, nu);
+    return s + (wrap_add(cm, a, pick4(g.L, mu)) - cm) * pick4(g.st, mu) +
+           (wrap_add(cn, b, pick4(g.L, nu)) - cn) * pick4(g.st, nu);
+}
+
+// Rectangular<2>::StapleRect (rectangular_action.hpp:186-225): for nu = mu+1, mu+2,
+// mu+3 (mod 4) the six 1x2 staples of link (x, mu) -- orthogonal to it (two), extending
+// it forward (two), backward (two) -- in the reference's grouping and order.
+__device__ __forceinline__ M3 staple_rect(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
+                                          int par)
+{
+    const int q = par ^ 1;
+    M3 st = m3_zero();
+#pragma unroll 1
+    for (int off = 1; off < 4; ++off) {
+        const int nu = (mu + off) & 3;
+        auto at = [&](int a, int b) { return shift2(g, s, c, mu, a, nu, b); };
+        const int nup = at(0, 1), nupp = at(0, 2), mup_nup = at(1, 1), mup = at(1, 0);
+        const int nud = at(0, -1), nudd = at(0, -2), mup_nudd = at(1, -2), mup_nud = at(1, -1);
+        // orthogonal: U_nu(x) U_nu(x+nu) U_mu(x+2nu) U_nu(x+mu+nu)^+ U_nu(x+mu)^+ and its mirror
+        {
+            M3 a = mul(mul(ldu(U, g, s, nu, par), ldu(U, g, nup, nu, q)), ldu(U, g, nupp, mu, par));
+            a = mul_nd(mul_nd(a, ldu(U, g, mup_nup, nu, par)), ldu(U, g, mup, nu, q));
+            M3 b = mul_dn(ldu(U, g, nud, nu, q), dag(ldu(U, g, nudd, nu, par)));
+            b = mul(mul(mul(b, ldu(U, g, nudd, mu, par)), ldu(U, g, mup_nudd, nu, q)), ldu(U, g, mup_nud, nu, par));
+            acc(st, add(a, b));
+        }
+        // forward: [U_nu(x) U_mu(x+nu) U_mu(x+mu+nu) U_nu(x+2mu)^+ + mirror] U_mu(x+mu)^+
+        {
+            const int mupp = at(2, 0), mupp_nud = at(2, -1);
+            M3 a = mul_nd(mul(mul(ldu(U, g, s, nu, par), ldu(U, g, nup, mu, q)), ldu(U, g, mup_nup, mu, par)),
+                          ldu(U, g, mupp, nu, par));
+            M3 b = mul(mul(mul_dn(ldu(U, g, nud, nu, q), ldu(U, g, nud, mu, q)), ldu(U, g, mup_nud, mu, par)),
+                       ldu(U, g, mupp_nud, nu, q));
+            acc(st, mul_nd(add(a, b), ldu(U, g, mup, mu, q)));
+        }
+        // backward: U_mu(x-mu)^+ [U_nu(x-mu) U_mu(x-mu+nu) U_mu(x+nu) U_nu(x+mu)^+ + mirror]
+        {
+            const int mud = at(-1, 0), mud_nup = at(-1, 1), mud_nud = at(-1, -1);
+            M3 a = mul_nd(mul(mul(ldu(U, g, mud, nu, q), ldu(U, g, mud_nup, mu, par)), ldu(U, g, nup, mu, q)),
+                          ldu(U, g, mup, nu, q));
+            M3 b = mul(mul(mul_dn(ldu(U, g, mud_nud, nu, par), ldu(U, g, mud_nud, mu, par)), ldu(U, g, nud, mu, q)),
+                       ldu(U, g, mup_nud, nu, par));
+            acc(st, mul_dn(ldu(U, g, mud, mu, q), add(a, b)));
+        }
+    }
+    return st;
+}
+
+// Rectangular<R>::Staple (rectangular_action.hpp:232-242)
+__device__ __forceinline__ M3 staple_action(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
+                                            int par, double c_plaq, double c_rect)
+{
+    const M3 p = staple(U, g, s, c, mu);
+    if (c_rect == 0.0) return c_plaq == 1.0 ? p : scale(c_plaq, p);
+    return add(scale(c_plaq, p), scale(c_rect, staple_rect(U, g, s, c, mu, par)));
+}
+
 // Sum of the four clover leaves of plane (mu, nu) at site s (leaf orientation of
 // the reference's CalculateClover<1,1>); leaf I (the plaquette) is returned too.
 __device__ __forceinline__ M3 clover_plane(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int par,

@@ -216,6 +216,9 @@ void k_bmw_pack(lt_field f, int K, long long n0, long long count, uint64_t* d_ou
 void k_bmw_unpack(lt_field f, int K, long long n0, long long count, const uint64_t* d_in, uint64_t* d_part,
                   double* d_dev);
 void k_parity(lt_field f, const double2* copy);
+void k_staple_field_action(lt_field u, lt_field out, double c_plaq, double c_rect);
+void k_or_sweep_action(lt_field u, int n, double c_plaq, double c_rect);
+void k_heatbath_sweep_action(lt_field u, double beta, int n, double c_plaq, double c_rect);
 void k_observables(lt_field u, double* host_out);
 void k_rect_sums(lt_field u, double out[3]);
 double k_wilson_loop(lt_field u, lt_field chain, int n0, int n1, int reset);

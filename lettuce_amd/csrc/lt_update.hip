@@ -157,6 +157,56 @@ __global__ void LT_LINK_KERNEL sweep_cb4_kernel(double2* __restrict__ U, Geom g,
     update_link<HEATBATH>(U, g, s, c, mu, parity_of(c), prefactor, seed, counter);
 }
 
+// Checkerboard4 sweeps with Rectangular<R>(beta, c_plaq, c_rect): the action enters
+// through the staple only (heatbath.hpp:110-129, overrelaxation.hpp:91-116)
+template <bool HEATBATH>
+__device__ __forceinline__ void update_link_action(double2* __restrict__ U, const Geom& g, int s, const int c[4],
+                                                   int mu, int par, double prefactor, uint64_t seed, uint64_t counter,
+                                                   double c_plaq, double c_rect)
+{
+    const M3 st = staple_action(U, g, s, c, mu, par, c_plaq, c_rect);
+    M3 L = ldu(U, g, s, mu, par);
+#pragma unroll
+    for (int sub = 0; sub < 3; ++sub) {
+        const int i = sub == 2 ? 1 : 0;
+        const int j = sub == 0 ? 1 : 2;
+        SU2 A = extract(L, st, i, j);
+        SU2 X;
+        if (HEATBATH)
+            X = heatbath_su2(A, prefactor, seed, (uint32_t)s, (uint32_t)sub, (uint32_t)mu, counter);
+        else
+            X = overrelax(A);
+        embed_mul(L, X, i, j);
+    }
+    gram_schmidt(L);
+    stu(U, g, s, mu, par, L);
+}
+
+template <bool HEATBATH>
+__global__ void __launch_bounds__(BS) sweep_action_kernel(double2* __restrict__ U, Geom g, int mu, int colour,
+                                                          double prefactor, uint64_t seed, uint64_t counter,
+                                                          double c_plaq, double c_rect)
+{
+    int s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (s >= g.V) return;
+    int c[4];
+    coords_of(g, s, c);
+    if (!cb4_member(c, mu, colour)) return;
+    update_link_action<HEATBATH>(U, g, s, c, mu, parity_of(c), prefactor, seed, counter, c_plaq, c_rect);
+}
+
+__global__ void __launch_bounds__(BS) staple_action_kernel(const double2* __restrict__ U, double2* __restrict__ out,
+                                                           Geom g, double c_plaq, double c_rect)
+{
+    const long long id = (long long)blockIdx.x * BS + threadIdx.x;
+    if (id >= 4LL * g.V) return;
+    const int s = (int)(id >> 2), mu = (int)(id & 3);
+    int c[4];
+    coords_of(g, s, c);
+    const int par = parity_of(c);
+    st_link(out, g, s, mu, par, staple_action(U, g, s, c, mu, par, c_plaq, c_rect));
+}
+
 __global__ void LT_LINK_KERNEL staple_field_kernel(const double2* __restrict__ U, double2* __restrict__ out, Geom g,
                                                    LinkMap m)
 {
@@ -188,7 +238,37 @@ void sweeps(lt_field u, int n, int schedule, double prefactor, const char* name)
     }
 }
 
+template <bool HEATBATH>
+void sweeps_action(lt_field u, int n, double prefactor, double c_plaq, double c_rect, const char* name)
+{
+    lt_ctx ctx = u->ctx;
+    const Geom& g = u->g;
+    for (int k = 0; k < n; ++k) {
+        uint64_t counter = HEATBATH ? u->counter++ : 0;
+        for (int mu = 0; mu < 4; ++mu)
+            for (int col = 0; col < 4; ++col)
+                launch(ctx, name, sweep_action_kernel<HEATBATH>, dim3(nblocks(g.V, BS)), dim3(BS), 0, u->d, g, mu, col,
+                       prefactor, u->seed, counter, c_plaq, c_rect);
+    }
+}
+
 }  // namespace
+
+void k_staple_field_action(lt_field u, lt_field out, double c_plaq, double c_rect)
+{
+    launch(u->ctx, "staple_action", staple_action_kernel, dim3(nblocks(4LL * u->g.V, BS)), dim3(BS), 0,
+           (const double2*)u->d, out->d, u->g, c_plaq, c_rect);
+}
+
+void k_or_sweep_action(lt_field u, int n, double c_plaq, double c_rect)
+{
+    sweeps_action<false>(u, n, 0.0, c_plaq, c_rect, "or_action");
+}
+
+void k_heatbath_sweep_action(lt_field u, double beta, int n, double c_plaq, double c_rect)
+{
+    sweeps_action<true>(u, n, 3.0 / beta, c_plaq, c_rect, "hb_action");
+}
 
 void k_staple_field(lt_field u, lt_field out)
 {

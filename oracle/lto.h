@@ -51,6 +51,10 @@ double lto_topcharge_plaquette(const double *U, lto_dims d);
 double lto_wilson_loop(const double *U, double *chain, lto_dims d, int n0, int n1, int reset);
 void   lto_polyakov_loop(const double *U, lto_dims d, double *re, double *im);
 void   lto_parity_update(const double *U, double *out, lto_dims d);
+void   lto_staple_action_field(const double *U, lto_dims d, double c_plaq, double c_rect, double *ST);
+void   lto_or_sweep_action(double *U, lto_dims d, double c_plaq, double c_rect, int n_sweep, int schedule);
+void   lto_hb_sweep_action(double *U, lto_dims d, double beta, double c_plaq, double c_rect, int n_sweep, int schedule,
+                           uint64_t seed, uint64_t *counter);
 void   lto_algebra(const double *M, double *out);           /* traceless anti-hermitian part */
 
 /* ---- actions / updates --------------------------------------------------- */

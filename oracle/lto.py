@@ -72,6 +72,10 @@ def _load():
         "lto_wilson_loop": (ctypes.c_double, [D, D, dims, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "lto_polyakov_loop": (None, [D, dims, D, D]),
         "lto_parity_update": (None, [D, D, dims]),
+        "lto_staple_action_field": (None, [D, dims, ctypes.c_double, ctypes.c_double, D]),
+        "lto_or_sweep_action": (None, [D, dims, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+        "lto_hb_sweep_action": (None, [D, dims, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -512,3 +516,26 @@ def parity_update(U, shape):
     out = new_field(shape)
     lib().lto_parity_update(_p(U), _p(out), dims(shape))
     return out
+
+
+# improved gauge actions Rectangular<2> (rectangular_action.hpp:29-300)
+LUESCHER_WEISZ = (1.0 + 8.0 / 12.0, -1.0 / 12.0)
+IWASAKI = (1.0 + 8.0 * 0.331, -0.331)
+DBW2 = (1.0 + 8.0 * 1.4088, -1.4088)
+
+
+def staple_action_field(U, shape, c_plaq, c_rect):
+    ST = np.zeros_like(U)
+    lib().lto_staple_action_field(_p(U), dims(shape), c_plaq, c_rect, _p(ST))
+    return ST
+
+
+def or_sweep_action(U, shape, c_plaq, c_rect, n=1, schedule=4):
+    lib().lto_or_sweep_action(_p(U), dims(shape), c_plaq, c_rect, n, schedule)
+    return U
+
+
+def hb_sweep_action(U, shape, beta, c_plaq, c_rect, n=1, schedule=4, seed=0, counter=0):
+    c = ctypes.c_uint64(counter)
+    lib().lto_hb_sweep_action(_p(U), dims(shape), beta, c_plaq, c_rect, n, schedule, seed, ctypes.byref(c))
+    return c.value

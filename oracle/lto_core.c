@@ -200,6 +200,60 @@ mat3 lto_staple_m(const double *U, lto_dims d, size_t s, int mu)
     return st;
 }
 
+/* Rectangular<2>::StapleRect (rectangular_action.hpp:186-225): for nu = mu+1, mu+2, mu+3
+ * (mod 4) the six 1x2 staples of the link (x, mu) -- two orthogonal to it, two
+ * extending it forward, two backward -- each the path from x+mu to x.  The
+ * reference leaves the accumulator uninitialised (Eigen); it starts from zero here. */
+static mat3 staple_rect_m(const double *U, lto_dims d, size_t s, int mu)
+{
+    mat3 st = m_zero();
+    for (int off = 1; off < 4; ++off) {
+        const int nu = (mu + off) % 4;
+        size_t nup = lto_move(d, s, nu, 1), nupp = lto_move(d, nup, nu, 1), mup_nup = lto_move(d, nup, mu, 1);
+        size_t mup = lto_move(d, s, mu, 1);
+        size_t nud = lto_move(d, s, nu, -1), nudd = lto_move(d, nud, nu, -1), mup_nudd = lto_move(d, nudd, mu, 1);
+        size_t mup_nud = lto_move(d, nud, mu, 1);
+        /* orthogonal: U_nu(x) U_nu(x+nu) U_mu(x+2nu) U_nu(x+mu+nu)^+ U_nu(x+mu)^+ and its mirror */
+        mat3 a = m_mul(m_mul(lto_link(U, s, nu), lto_link(U, nup, nu)), lto_link(U, nupp, mu));
+        a = m_mul_nd(m_mul_nd(a, lto_link(U, mup_nup, nu)), lto_link(U, mup, nu));
+        mat3 b = m_mul(m_mul_dn(lto_link(U, nud, nu), m_dag(lto_link(U, nudd, nu))), lto_link(U, nudd, mu));
+        b = m_mul(m_mul(b, lto_link(U, mup_nudd, nu)), lto_link(U, mup_nud, nu));
+        st = m_add(st, m_add(a, b));
+        /* forward: [U_nu(x) U_mu(x+nu) U_mu(x+mu+nu) U_nu(x+2mu)^+ + mirror] U_mu(x+mu)^+ */
+        size_t mupp = lto_move(d, mup, mu, 1), mupp_nud = lto_move(d, mup_nud, mu, 1);
+        mat3 c = m_mul_nd(m_mul(m_mul(lto_link(U, s, nu), lto_link(U, nup, mu)), lto_link(U, mup_nup, mu)),
+                          lto_link(U, mupp, nu));
+        mat3 e = m_mul(m_mul(m_mul_dn(lto_link(U, nud, nu), lto_link(U, nud, mu)), lto_link(U, mup_nud, mu)),
+                       lto_link(U, mupp_nud, nu));
+        st = m_add(st, m_mul_nd(m_add(c, e), lto_link(U, mup, mu)));
+        /* backward: U_mu(x-mu)^+ [U_nu(x-mu) U_mu(x-mu+nu) U_mu(x+nu) U_nu(x+mu)^+ + mirror] */
+        size_t mud = lto_move(d, s, mu, -1), mud_nup = lto_move(d, mud, nu, 1), mud_nud = lto_move(d, mud, nu, -1);
+        mat3 f = m_mul_nd(m_mul(m_mul(lto_link(U, mud, nu), lto_link(U, mud_nup, mu)), lto_link(U, nup, mu)),
+                          lto_link(U, mup, nu));
+        mat3 h = m_mul(m_mul(m_mul_dn(lto_link(U, mud_nud, nu), lto_link(U, mud_nud, mu)), lto_link(U, nud, mu)),
+                       lto_link(U, mup_nud, nu));
+        st = m_add(st, m_mul_dn(lto_link(U, mud, mu), m_add(f, h)));
+    }
+    return st;
+}
+
+/* Rectangular<R>::Staple (rectangular_action.hpp:232-242): c_plaq StaplePlaq + c_rect
+ * StapleRect, or StaplePlaq alone for c_rect = 0 (the Wilson action, c_plaq = 1) */
+static mat3 staple_action_m(const double *U, lto_dims d, size_t s, int mu, double c_plaq, double c_rect)
+{
+    mat3 p = lto_staple_m(U, d, s, mu);
+    if (c_rect == 0.0) return c_plaq == 1.0 ? p : m_rscale(c_plaq, p);
+    return m_add(m_rscale(c_plaq, p), m_rscale(c_rect, staple_rect_m(U, d, s, mu)));
+}
+
+void lto_staple_action_field(const double *U, lto_dims d, double c_plaq, double c_rect, double *ST)
+{
+    size_t V = lto_volume(d);
+    #pragma omp parallel for
+    for (size_t s = 0; s < V; ++s)
+        for (int mu = 0; mu < 4; ++mu) lto_set_link(ST, s, mu, staple_action_m(U, d, s, mu, c_plaq, c_rect));
+}
+
 void lto_staple(const double *U, lto_dims d, size_t site, int mu, double *st)
 {
     m_store(st, lto_staple_m(U, d, site, mu));
@@ -304,10 +358,13 @@ static su2 su2_heatbath(su2 A, double prefactor, int max_iter, uint64_t seed, ui
     return su2_mul(X, Vd);
 }
 
+/* the gauge action of the sweeps: (c_plaq, c_rect) of Rectangular<R> */
+static double g_c_plaq = 1.0, g_c_rect = 0.0;
+
 /* OverrelaxationSubgroupKernel::operator() (overrelaxation.hpp:91-116) */
 static void or_link(double *U, lto_dims d, size_t s, int mu)
 {
-    mat3 st_adj = m_dag(lto_staple_m(U, d, s, mu));
+    mat3 st_adj = m_dag(staple_action_m(U, d, s, mu, g_c_plaq, g_c_rect));
     mat3 L = lto_link(U, s, mu);
     for (int g = 0; g < 3; ++g) {
         su2 A = su2_extract(m_mul(L, st_adj), g);
@@ -321,7 +378,7 @@ static void or_link(double *U, lto_dims d, size_t s, int mu)
 static void hb_link(double *U, lto_dims d, size_t s, int mu, double prefactor, int max_iter,
                     uint64_t seed, uint64_t counter, int *attempts)
 {
-    mat3 st_adj = m_dag(lto_staple_m(U, d, s, mu));
+    mat3 st_adj = m_dag(staple_action_m(U, d, s, mu, g_c_plaq, g_c_rect));
     mat3 L = lto_link(U, s, mu);
     for (int g = 0; g < 3; ++g) {
         su2 A = su2_extract(m_mul(L, st_adj), g);
@@ -372,6 +429,27 @@ void lto_hb_sweep(double *U, lto_dims d, double beta, int n_sweep, int schedule,
                         hb_link(U, d, s, mu, prefactor, 10, seed, c, NULL);
             }
     }
+}
+
+/* Sweeps with Rectangular<R>(beta, c_plaq, c_rect) (rectangular_action.hpp:29-60):
+ * the action only enters through the staple; R = 2 needs the four-colour schedule. */
+void lto_or_sweep_action(double *U, lto_dims d, double c_plaq, double c_rect, int n_sweep, int schedule)
+{
+    g_c_plaq = c_plaq;
+    g_c_rect = c_rect;
+    lto_or_sweep(U, d, n_sweep, schedule);
+    g_c_plaq = 1.0;
+    g_c_rect = 0.0;
+}
+
+void lto_hb_sweep_action(double *U, lto_dims d, double beta, double c_plaq, double c_rect, int n_sweep, int schedule,
+                         uint64_t seed, uint64_t *counter)
+{
+    g_c_plaq = c_plaq;
+    g_c_rect = c_rect;
+    lto_hb_sweep(U, d, beta, n_sweep, schedule, seed, counter);
+    g_c_plaq = 1.0;
+    g_c_rect = 0.0;
 }
 
 void lto_hb_link(double *U, lto_dims d, double beta, size_t site, int mu, int max_iter,

@@ -428,6 +428,22 @@ int main(int argc, char** argv)
         out.push_back(std::real(P));
         out.push_back(std::imag(P));
         write_values(argv[3], out);
+    } else if (cmd == "staple_action") {         // staple_action in out c_plaq c_rect: Rectangular<2>::Staple
+        read_field(argv[2], *U);
+        GaugeAction::Rectangular<2> act(6.0, std::atof(argv[4]), std::atof(argv[5]));
+        #pragma omp parallel for
+        for (std::size_t s = 0; s < kLinks / 4; ++s)
+            for (int mu = 0; mu < 4; ++mu) {
+                int z = s % Nz, y = (s / Nz) % Ny, x = (s / Nz / Ny) % Nx, t = s / Nz / Ny / Nx;
+                (*U2)({t, x, y, z, mu}) = act.Staple(*U, {t, x, y, z, mu});
+            }
+        write_field(argv[3], *U2);
+    } else if (cmd == "or_action") {             // or_action in out c_plaq c_rect n: Checkerboard4 OR, Rectangular<2>
+        read_field(argv[2], *U);
+        GaugeAction::Rectangular<2> act(6.0, std::atof(argv[4]), std::atof(argv[5]));
+        OverrelaxationSubgroupKernel ORk(*U, act);
+        Iterator::Checkerboard4(ORk, std::atoi(argv[6]));
+        write_field(argv[3], *U);
     } else if (cmd == "parity") {                // parity in out: ParityUpdate (parity_update.hpp:22-47)
         read_field(argv[2], *U);
         ParityUpdate(*U, *U2);
