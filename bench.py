@@ -340,6 +340,32 @@ def main():
     obs_rest_ms = (time.perf_counter() - t0) / 3 * 1e3
     chain.close()
     fl.close()
+    # improved action (Luescher-Weisz) Checkerboard4 sweeps on the flowed copy
+    W.assign(U)
+    lw_or = la.OverrelaxationSubgroupKernel(W, action=la.LuescherWeiszAction(args.beta))
+    la.Iterator.Checkerboard4(lw_or, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    la.Iterator.Checkerboard4(lw_or, 2)
+    ctx.synchronize()
+    lw_or_ms = (time.perf_counter() - t0) / 2 * 1e3
+    # BMW files written and read back through the device codec (#18BMW, all rows)
+    io = {}
+    try:
+        import tempfile
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "config.conf")
+            t0 = time.perf_counter()
+            W.save_bmw(path, full=True)
+            t_save = time.perf_counter() - t0
+            size = os.path.getsize(path)
+            t0 = time.perf_counter()
+            ok, _ = W.load_bmw(path, full=True)
+            t_load = time.perf_counter() - t0
+        io = {"format": "#18BMW", "bytes": size, "save_s": t_save, "load_s": t_load, "checksum_ok": ok,
+              "save_GBs": size / t_save / 1e9, "load_GBs": size / t_load / 1e9}
+    except (OSError, la.LettuceError) as e:
+        io = {"error": str(e)}
     W.close()
 
     result = {
@@ -373,6 +399,8 @@ def main():
                      "link_stage_updates_per_s": 12 * V / (flow_ms * 1e-3)},
         "observables_pass_ms": obs_ms,
         "observables_rest_ms": obs_rest_ms,
+        "lw_or_sweep_ms": lw_or_ms,
+        "bmw_io": io,
         "plaquette": plaq,
         "thermalisation_s": t_therm,
     }
