@@ -6,6 +6,11 @@
 
 #include "lt_device.hpp"
 
+// the MetaD stout tape kernel with the software-pipelined staple
+#ifndef LT_MD_PIPE
+#define LT_MD_PIPE 1
+#endif
+
 namespace lt {
 
 // ---------------------------------------------------------------- HMC
@@ -159,8 +164,14 @@ __device__ __forceinline__ void body_stout_tape(const double2* __restrict__ Uin,
 {
     int mu, par, c[4], s;
     if (!map_link_b(g, m, b, mu, par, c, s)) return;
+#if LT_MD_PIPE
+    Raw6 own;
+    const M3 st = staple_pipelined(Uin, g, s, c, mu, par, own);
+    const M3 L = su3_of(own);
+#else
     M3 st = staple(Uin, g, s, c, mu);
     M3 L = ldu(Uin, g, s, mu, par);
+#endif
     M3 A = algebra(mul_nd(st, L));
     M3 Q;
 #pragma unroll
