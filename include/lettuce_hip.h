@@ -182,6 +182,34 @@ int lt_flow_resume(lt_flow fl, int n_steps);
 #define LT_INT_LEAPFROG 1
 #define LT_INT_OMF2     2
 #define LT_INT_OMF4     4
+/* Multiple-timescale MetaD integrators, Integrators::HMC::{Leapfrog_OMF_4,
+ * OMF_2_OMF_4_slow, OMF_2_OMF_4} (hmc_metadynamics.hpp:28-238): the bias force
+ * kicks on the outer scale (leapfrog or OMF2), the gauge force on an inner OMF4.
+ * MetaD-HMC only (lt_metad_hmc_trajectory, lt_batch_metad_hmc).  Each outer kick
+ * evaluates the bias force at the current field (UpdateMetadynamicsMomenta with a
+ * fresh CalculateTopologicalForce; the reference's member reuses the last force
+ * its UpdateMomenta computed, which these integrators never call).  The merged
+ * outer kicks, and the last one, record the CV path samples, n_step per trajectory
+ * (OMF_2_OMF_4_slow: the last kick of each step).  OMF_2_OMF_4_slow is the
+ * reference's schedule as written: its inner OMF4 blocks advance the field by 2 eps
+ * per step while its outer kicks add up to eps. */
+#define LT_INT_LEAPFROG_OMF4  5
+#define LT_INT_OMF2_OMF4_SLOW 6
+#define LT_INT_OMF2_OMF4      7
+/* Integrators::HMC::{Leapfrog_slow, OMF_2_slow, OMF_4_slow} (hmc_gauge.hpp:21-46,
+ * 121-153, 195-237): the single-scale schedules without merged momentum updates */
+#define LT_INT_LEAPFROG_SLOW  8
+#define LT_INT_OMF2_SLOW      9
+#define LT_INT_OMF4_SLOW      10
+
+/* The kick / drift sequence of an integrator (no reference counterpart: there each
+ * integrator's operator() issues it to the HMC kernel).  Event i: kinds[i] = 'P'
+ * (momentum update with both forces, UpdateMomenta), 'G' (gauge force only), 'M'
+ * (bias force only) or 'U' (UpdateFields), eps[i] its step size, sub[i] = 1 where the
+ * MetaD path update samples the CV.  *n = the number of events; at most cap are
+ * written (the arrays may be null when cap = 0).  Host only. */
+int lt_integrator_schedule(int integrator, double tau, int n_step, char *kinds, double *eps, int *sub, int cap,
+                           int *n);
 
 /* a field of general 3x3 matrices (momenta, forces): all nine entries stored */
 int lt_matrix_field_create(lt_ctx ctx, int nt, int nx, int ny, int nz, lt_field *out);
@@ -263,6 +291,15 @@ int lt_bias_info(lt_bias b, double *params, int *bins, unsigned long long *excee
 int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F, lt_cv cv, lt_bias bias,
                             double beta, int integrator, double tau, int n_step, int metropolis,
                             int update_bias, double *dH, int *accepted, double *cv_new);
+/* cv_path_samples of the last lt_metad_hmc_trajectory on cv (hmc_metadynamics.hpp:398-404):
+ * deposited = 0 the CVs visited on the path, else the values deposited (CV_old repeated
+ * after a rejection, hmc_metadynamics.hpp:632-638); *n = their number, at most cap copied */
+int lt_cv_path_samples(lt_cv cv, int deposited, double *out, int cap, int *n);
+/* HMCMetaDKernel::LogCVData (hmc_metadynamics.hpp:537-545): appends those samples to
+ * the file as one line, separated by blanks, in the default stream format; the
+ * reference logs the visited values after every trajectory and, with the Metropolis
+ * step, the deposited ones after it (hmc_metadynamics.hpp:613,642) */
+int lt_cv_log_path(lt_cv cv, int deposited, const char *path);
 
 /* ---- replica batches and parallel-tempering metadynamics ------------------ */
 /* The reference tempers two streams, an unbiased one and one MetaD-HMC stream

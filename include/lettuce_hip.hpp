@@ -242,10 +242,13 @@ class CollectiveVariable {
 // (updates/hmc_metadynamics.hpp:264-675) with the hand-written adjoint force
 class HMCMetaDKernel {
   public:
+    // hmc_log_path: where each trajectory appends its CV path (LogCVData,
+    // hmc_metadynamics.hpp:537-545); empty = no log
     HMCMetaDKernel(GaugeField& U, GaugeField& U_copy, MatrixField& P, MatrixField& F, MetaBiasPotential& bias,
-                   CollectiveVariable& cv, int integrator, double beta, double tau, bool update_bias = true)
+                   CollectiveVariable& cv, int integrator, double beta, double tau, bool update_bias = true,
+                   std::string hmc_log_path = {})
         : U_(U), Uc_(U_copy), P_(P), F_(F), bias_(bias), cv_(cv), integrator_(integrator), beta_(beta), tau_(tau),
-          update_bias_(update_bias)
+          update_bias_(update_bias), log_(std::move(hmc_log_path))
     {
     }
     bool operator()(int n_step, bool metropolis_step = true)
@@ -255,6 +258,10 @@ class HMCMetaDKernel {
                                                  bias_.handle(), beta_, integrator_, tau_, n_step,
                                                  metropolis_step ? 1 : 0, update_bias_ ? 1 : 0, &delta_H, &acc,
                                                  &cv_value));
+        if (!log_.empty() && n_step > 0) {
+            LETTUCE_HIP_CALL(lt_cv_log_path(cv_.handle(), 0, log_.c_str()));
+            if (metropolis_step) LETTUCE_HIP_CALL(lt_cv_log_path(cv_.handle(), 1, log_.c_str()));
+        }
         return acc != 0;
     }
     double delta_H = 0.0, cv_value = 0.0;
@@ -269,6 +276,7 @@ class HMCMetaDKernel {
     int integrator_;
     double beta_, tau_;
     bool update_bias_;
+    std::string log_;
 };
 
 // Replica batch: the reference's two-stream MetadynamicsTemperingKernel

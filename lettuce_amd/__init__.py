@@ -35,7 +35,8 @@ __all__ = [
     "CB2", "CB4", "SaveConfigBMW", "SaveConfigBMWFull", "LoadConfigBMW", "LoadConfigBMWFull", "CheckpointManager",
     "PhiloxState", "RectangularAction", "TopChargePlaquette", "WilsonLoop", "PolyakovLoop",
     "ParityUpdate", "MetadynamicsTemperingKernel", "LuescherWeiszAction", "IwasakiAction", "DBW2Action",
-    "StapleAction",
+    "StapleAction", "IntegratorSchedule", "HMCKernel", "HMCMetaDKernel", "MetaBiasPotential",
+    "CollectiveVariable", "MatrixField", "ReplicaBatch",
 ]
 
 CB2 = 2
@@ -426,7 +427,30 @@ class GradientFlowKernel:
 LEAPFROG = 1
 OMF2 = 2
 OMF4 = 4
-_INTEGRATORS = {"leapfrog": LEAPFROG, "omf2": OMF2, "omf4": OMF4}
+# multiple-timescale MetaD integrators (hmc_metadynamics.hpp:28-238), MetaD-HMC only
+LEAPFROG_OMF4 = 5
+OMF2_OMF4_SLOW = 6
+OMF2_OMF4 = 7
+# without merged momentum updates (hmc_gauge.hpp:21-46,121-153,195-237)
+LEAPFROG_SLOW = 8
+OMF2_SLOW = 9
+OMF4_SLOW = 10
+_INTEGRATORS = {"leapfrog": LEAPFROG, "omf2": OMF2, "omf4": OMF4, "leapfrog_omf4": LEAPFROG_OMF4,
+                "omf2_omf4_slow": OMF2_OMF4_SLOW, "omf2_omf4": OMF2_OMF4, "leapfrog_slow": LEAPFROG_SLOW,
+                "omf2_slow": OMF2_SLOW, "omf4_slow": OMF4_SLOW}
+
+
+def IntegratorSchedule(integrator, tau, n_step):
+    """The kick / drift events of an integrator as [(kind, eps, sub)]: kind 'P' (both
+    forces), 'G' (gauge force), 'M' (bias force) or 'U' (field update)."""
+    code = _INTEGRATORS[integrator]
+    n = ctypes.c_int()
+    call("lt_integrator_schedule", code, tau, n_step, None, None, None, 0, ctypes.byref(n))
+    kinds = ctypes.create_string_buffer(max(n.value, 1))
+    eps = np.zeros(max(n.value, 1))
+    sub = (ctypes.c_int * max(n.value, 1))()
+    call("lt_integrator_schedule", code, tau, n_step, kinds, _dp(eps), sub, n.value, ctypes.byref(n))
+    return [(kinds.raw[i:i + 1].decode(), float(eps[i]), int(sub[i])) for i in range(n.value)]
 
 
 class MatrixField(GaugeField):
@@ -613,15 +637,22 @@ class MetaBiasPotential:
 
 
 class HMCMetaDKernel:
-    """GaugeUpdates::HMCMetaDKernel (hmc_metadynamics.hpp:264-675) with the adjoint force."""
+    """GaugeUpdates::HMCMetaDKernel (hmc_metadynamics.hpp:264-675) with the adjoint force.
+
+    integrator: "leapfrog", "omf2", "omf4" (both forces in every kick) or the
+    multiple-timescale "leapfrog_omf4", "omf2_omf4_slow", "omf2_omf4" (bias force on
+    the outer scale, gauge force on an inner OMF4).  With hmc_log_path set, every
+    trajectory appends its CV path to that file as LogCVData does (the visited values;
+    with the Metropolis step a second line with the values deposited)."""
 
     def __init__(self, U, U_copy, P, F, bias, cv, integrator="omf2", beta=6.0, trajectory_length=1.0,
-                 update_bias=True):
+                 update_bias=True, hmc_log_path=None):
         self.U, self.U_copy, self.P, self.F, self.bias, self.cv = U, U_copy, P, F, bias, cv
         self.integrator = _INTEGRATORS[integrator]
         self.beta = beta
         self.trajectory_length = trajectory_length
         self.update_bias = update_bias
+        self.hmc_log_path = hmc_log_path
         self.delta_H = 0.0
         self.cv_value = 0.0
 
@@ -635,7 +666,23 @@ class HMCMetaDKernel:
              ctypes.byref(q))
         self.delta_H = d.value
         self.cv_value = q.value
+        if self.hmc_log_path is not None and n_step > 0:
+            self.LogCVData(False)
+            if metropolis_step:
+                self.LogCVData(True)
         return bool(a.value)
+
+    def path_samples(self, deposited=False):
+        """cv_path_samples of the last trajectory: the visited CVs, or the deposited values."""
+        n = ctypes.c_int()
+        call("lt_cv_path_samples", self.cv.handle, 1 if deposited else 0, None, 0, ctypes.byref(n))
+        out = np.zeros(n.value)
+        call("lt_cv_path_samples", self.cv.handle, 1 if deposited else 0, _dp(out), n.value, ctypes.byref(n))
+        return out
+
+    def LogCVData(self, deposited=False):
+        """HMCMetaDKernel::LogCVData (hmc_metadynamics.hpp:537-545): one line appended to hmc_log_path."""
+        call("lt_cv_log_path", self.cv.handle, 1 if deposited else 0, str(self.hmc_log_path).encode())
 
 
 class MetadynamicsTemperingKernel:

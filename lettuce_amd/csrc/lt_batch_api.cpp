@@ -117,7 +117,8 @@ using lt::guarded;
 int n_sub_of(int integrator, double tau, int n_step)
 {
     int n_sub = 0;
-    lt::hmc_schedule(integrator, tau, n_step, [&](double, bool sub) { n_sub += sub ? 1 : 0; }, [](double) {});
+    lt::metad_schedule(integrator, tau, n_step, [&](double, bool sub, int) { n_sub += sub ? 1 : 0; },
+                       [](double) {});
     return n_sub;
 }
 
@@ -156,17 +157,24 @@ void enqueue_step(lt_batch b, const StepKey& p, int n_sub)
     lt::kb_kinetic(ctx, bg, b->total_kb, b->P, b->part, st.kin);
     lt::kb_h0(ctx, bg, st, bias, p.beta);
     int isub = 0;
-    lt::hmc_schedule(
+    lt::metad_schedule(
         p.integrator, p.tau, p.n_step,
-        [&](double e, bool sub) {
-            // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436) + UpdateMomenta (:441-459)
-            forward(b, bg);
-            lt::kb_dvdq(ctx, bg, st, bias, sub ? isub++ : -1);
-            lt::kb_clover_deriv(ctx, bg, b->total_lb, b->levels[b->n_levels], b->clover, b->F, st.dvdq);
-            for (int k = b->n_levels; k > 0; --k)
-                lt::kb_stout_recursion(ctx, bg, b->total_lb, b->levels[k - 1], b->levels[k], b->F, b->tapes[k - 1],
-                                       b->lambda, b->fsum, b->rho);
-            lt::kb_force_update(ctx, bg, b->total_lb, b->U, b->P, b->F, e * p.beta / 6.0, e);
+        [&](double e, bool sub, int kick) {
+            if (kick & lt::KICK_BIAS) {
+                // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436)
+                forward(b, bg);
+                lt::kb_dvdq(ctx, bg, st, bias, sub ? isub++ : -1);
+                lt::kb_clover_deriv(ctx, bg, b->total_lb, b->levels[b->n_levels], b->clover, b->F, st.dvdq);
+                for (int k = b->n_levels; k > 0; --k)
+                    lt::kb_stout_recursion(ctx, bg, b->total_lb, b->levels[k - 1], b->levels[k], b->F,
+                                           b->tapes[k - 1], b->lambda, b->fsum, b->rho);
+            }
+            if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)
+                lt::kb_force_update(ctx, bg, b->total_lb, b->U, b->P, b->F, e * p.beta / 6.0, e);
+            else if (kick == lt::KICK_GAUGE)   // UpdateGaugeMomenta (:464-477)
+                lt::kb_force_update(ctx, bg, b->total_lb, b->U, b->P, nullptr, e * p.beta / 6.0, 0.0);
+            else   // UpdateMetadynamicsMomenta (:479-491)
+                lt::k_axpy(ctx, b->F, b->P, e, (size_t)b->total);
         },
         [&](double e) { lt::kb_field_update(ctx, bg, b->total_lb, b->U, b->P, e); });
     forward(b, bg);
@@ -511,7 +519,7 @@ int lt_batch_use_graph(lt_batch b, int enable)
 int lt_batch_metad_hmc(lt_batch b, double beta, int integrator, double tau, int n_step, int metropolis,
                        int update_bias, int swap_mode, int n_traj)
 {
-    if (!b || !lt::valid_integrator(integrator) || n_step < 1 || n_traj < 0 || swap_mode < 0 || swap_mode > 3)
+    if (!b || !lt::valid_metad_integrator(integrator) || n_step < 1 || n_traj < 0 || swap_mode < 0 || swap_mode > 3)
         return fail(LT_EINVAL, "lt_batch_metad_hmc: bad arguments");
     if (!b->has_bias) return fail(LT_ESTATE, "lt_batch_metad_hmc: call lt_batch_set_bias first");
     return guarded([&] {

@@ -151,6 +151,20 @@ __global__ void __launch_bounds__(BS) copy_kernel(const double2* __restrict__ sr
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+__global__ void __launch_bounds__(BS) axpy_kernel(const double2* __restrict__ x, double2* __restrict__ y, double a,
+                                                  size_t n)
+{
+    size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * BS;
+    for (; i < n; i += stride) {
+        const double2 u = x[i];
+        double2 v = y[i];
+        v.x += a * u.x;
+        v.y += a * u.y;
+        y[i] = v;
+    }
+}
+
 }  // namespace
 
 void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n)
@@ -158,6 +172,13 @@ void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n)
     unsigned nb = nblocks((long long)n, BS);
     if (nb > 65536u) nb = 65536u;
     launch(ctx, "copy_links", copy_kernel, dim3(nb), dim3(BS), 0, src, dst, n);
+}
+
+void k_axpy(lt_ctx ctx, const double2* x, double2* y, double a, size_t n)
+{
+    unsigned nb = nblocks((long long)n, BS);
+    if (nb > 65536u) nb = 65536u;
+    launch(ctx, "axpy_links", axpy_kernel, dim3(nb), dim3(BS), 0, x, y, a, n);
 }
 
 void k_pack(lt_field f, const double* d_aos)

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <functional>
 #include <string>
 #include <vector>
@@ -54,6 +55,8 @@ struct lt_cv_s {
     double2* fsum = nullptr;        // six-term Lambda staple sums of the recursion
     size_t bytes = 0;               // device memory held
     bool forward_done = false;
+    std::vector<double> path;       // CVs visited by the last MetaD-HMC trajectory
+    std::vector<double> path_used;  // the values it deposited (CV_old repeated if rejected)
 
     void release()
     {
@@ -113,15 +116,24 @@ double host_uniform(uint64_t seed, uint64_t counter)
 
 }  // namespace
 
-// Integrators::HMC::{Leapfrog, OMF_2, OMF_4} (LettuceGauge/updates/hmc_gauge.hpp:48-75,
-// 155-190, 241-295); momentum updates carry the "submeasurement" flag the MetaD
-// path update uses.
+// Integrators::HMC::{Leapfrog, OMF_2, OMF_4} with merged momentum updates and
+// {Leapfrog_slow, OMF_2_slow, OMF_4_slow} without (LettuceGauge/updates/hmc_gauge.hpp:21-75,
+// 121-295), coefficients formed as there; momentum updates carry the "submeasurement"
+// flag the MetaD path update uses.
 void lt::hmc_schedule(int kind, double tau, int n_step, const std::function<void(double, bool)>& P,
                       const std::function<void(double)>& U)
 {
     if (n_step <= 0) return;
     const double eps = tau / n_step;
-    if (kind == LT_INT_LEAPFROG) {
+    if (kind == LT_INT_LEAPFROG || kind == LT_INT_LEAPFROG_SLOW) {
+        if (kind == LT_INT_LEAPFROG_SLOW) {
+            for (int k = 0; k < n_step; ++k) {
+                P(0.5 * eps, false);
+                U(eps);
+                P(0.5 * eps, true);
+            }
+            return;
+        }
         P(0.5 * eps, false);
         for (int k = 0; k < n_step - 1; ++k) {
             U(eps);
@@ -129,9 +141,19 @@ void lt::hmc_schedule(int kind, double tau, int n_step, const std::function<void
         }
         U(eps);
         P(0.5 * eps, true);
-    } else if (kind == LT_INT_OMF2) {
+    } else if (kind == LT_INT_OMF2 || kind == LT_INT_OMF2_SLOW) {
         const double lam = 0.1931833275037836;
         const double alpha = lam * eps, beta = 0.5 * eps, gamma = (1.0 - 2.0 * lam) * eps;
+        if (kind == LT_INT_OMF2_SLOW) {
+            for (int k = 0; k < n_step; ++k) {
+                P(alpha, false);
+                U(beta);
+                P(gamma, false);
+                U(beta);
+                P(alpha, true);
+            }
+            return;
+        }
         P(alpha, false);
         U(beta);
         P(gamma, false);
@@ -159,6 +181,14 @@ void lt::hmc_schedule(int kind, double tau, int n_step, const std::function<void
             P(gamma, false);
             U(beta);
         };
+        if (kind == LT_INT_OMF4_SLOW) {
+            for (int k = 0; k < n_step; ++k) {
+                P(alpha, false);
+                body();
+                P(alpha, true);
+            }
+            return;
+        }
         P(alpha, false);
         body();
         for (int k = 0; k < n_step - 1; ++k) {
@@ -167,6 +197,72 @@ void lt::hmc_schedule(int kind, double tau, int n_step, const std::function<void
         }
         P(alpha, true);
     }
+}
+
+// Integrators::HMC::{Leapfrog_OMF_4, OMF_2_OMF_4_slow, OMF_2_OMF_4} (hmc_metadynamics.hpp:28-238)
+// with the coefficients formed as there; the single-scale kinds kick both forces at once.
+void lt::metad_schedule(int kind, double tau, int n_step, const std::function<void(double, bool, int)>& P,
+                        const std::function<void(double)>& U)
+{
+    if (valid_integrator(kind)) {
+        hmc_schedule(kind, tau, n_step, [&](double e, bool sub) { P(e, sub, KICK_BOTH); }, U);
+        return;
+    }
+    if (n_step <= 0) return;
+    const double eps = tau / n_step;
+    // the inner OMF4 step; OMF_2_OMF_4 scales it by 1/2 (two inner steps per outer step)
+    const double f = kind == LT_INT_OMF2_OMF4 ? 0.5 : 1.0;
+    const double a4 = f * 0.08398315262876693 * eps, b4 = f * 0.2539785108410595 * eps;
+    const double g4 = f * 0.6822365335719091 * eps, d4 = f * -0.03230286765269967 * eps;
+    const double m4 = f * (0.5 - 0.6822365335719091 - 0.08398315262876693) * eps;
+    const double n4 = f * (1.0 - 2.0 * -0.03230286765269967 - 2.0 * 0.2539785108410595) * eps;
+    auto G = [&](double e) { P(e, false, KICK_GAUGE); };
+    auto inner = [&]() {
+        G(a4);
+        U(b4);
+        G(g4);
+        U(d4);
+        G(m4);
+        U(n4);
+        G(m4);
+        U(d4);
+        G(g4);
+        U(b4);
+        G(a4);
+    };
+    auto M = [&](double e, bool sub) { P(e, sub, KICK_BIAS); };
+    if (kind == LT_INT_LEAPFROG_OMF4) {
+        M(0.5 * eps, false);
+        for (int k = 0; k < n_step - 1; ++k) {
+            inner();
+            M(eps, true);
+        }
+        inner();
+        M(0.5 * eps, true);
+        return;
+    }
+    const double a2 = 0.1931833275037836 * eps, b2 = (1.0 - 2.0 * 0.1931833275037836) * eps;
+    if (kind == LT_INT_OMF2_OMF4_SLOW) {
+        for (int k = 0; k < n_step; ++k) {
+            M(a2, false);
+            inner();
+            M(b2, false);
+            inner();
+            M(a2, true);
+        }
+        return;
+    }
+    M(a2, false);
+    for (int k = 0; k < n_step - 1; ++k) {
+        inner();
+        M(b2, false);
+        inner();
+        M(2.0 * a2, true);
+    }
+    inner();
+    M(b2, false);
+    inner();
+    M(a2, true);
 }
 
 namespace {
@@ -544,8 +640,9 @@ int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F,
                             int integrator, double tau, int n_step, int metropolis, int update_bias, double* dH,
                             int* accepted, double* cv_new)
 {
-    if (!U || !P || !U_copy || !F || !cv || !bias || !valid_integrator(integrator) || n_step < 0 || U == U_copy ||
-        !same_shape(U->g, P->g) || !same_shape(U->g, U_copy->g) || !same_shape(U->g, F->g) || !same_shape(U->g, cv->g))
+    if (!U || !P || !U_copy || !F || !cv || !bias || !lt::valid_metad_integrator(integrator) || n_step < 0 ||
+        U == U_copy || !same_shape(U->g, P->g) || !same_shape(U->g, U_copy->g) || !same_shape(U->g, F->g) ||
+        !same_shape(U->g, cv->g))
         return fail(LT_EINVAL, "lt_metad_hmc_trajectory: bad arguments");
     return guarded([&] {
         const uint64_t counter = U->counter++;
@@ -557,16 +654,24 @@ int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F,
         const double H0 = hamiltonian(U, P, beta) + bias->potential(cv_old);
         std::vector<double> samples;
         samples.reserve(n_step);
-        integrate(
+        lt::metad_schedule(
             integrator, tau, n_step,
-            [&](double e, bool sub) {
-                // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436) + UpdateMomenta (:441-459)
-                const double q = cv_forward(cv, U);
-                if (sub) samples.push_back(q);
-                cv_backward(cv, bias->derivative(q), F->d);
-                lt::k_force_update(U, P, F->d, e * beta / 6.0, e);
+            [&](double e, bool sub, int kick) {
+                if (kick & lt::KICK_BIAS) {
+                    // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436)
+                    const double q = cv_forward(cv, U);
+                    if (sub) samples.push_back(q);
+                    cv_backward(cv, bias->derivative(q), F->d);
+                }
+                if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)
+                    lt::k_force_update(U, P, F->d, e * beta / 6.0, e);
+                else if (kick == lt::KICK_GAUGE)   // UpdateGaugeMomenta (:464-477)
+                    lt::k_force_update(U, P, nullptr, e * beta / 6.0, 0.0);
+                else   // UpdateMetadynamicsMomenta (:479-491)
+                    lt::k_axpy(U->ctx, F->d, P->d, e, (size_t)72 * P->g.Vh);
             },
             [&](double e) { lt::k_field_update(U, P, e); });
+        cv->path = samples;
         const double cv1 = cv_charge(cv, U);
         const double H1 = hamiltonian(U, P, beta) + bias->potential(cv1);
         const double d = H1 - H0;
@@ -582,6 +687,7 @@ int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F,
             std::fill(samples.begin(), samples.end(), cv_old);
             copy_field(U, U_copy);
         }
+        cv->path_used = samples;
         if (update_bias)
             for (double s : samples) {
                 bias->deposit(s);
@@ -592,6 +698,50 @@ int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F,
         if (cv_new) *cv_new = acc ? cv1 : cv_old;
         return LT_OK;
     });
+}
+
+int lt_integrator_schedule(int integrator, double tau, int n_step, char* kinds, double* eps, int* sub, int cap,
+                           int* n)
+{
+    if (!lt::valid_metad_integrator(integrator) || n_step < 0 || !n || cap < 0 ||
+        (cap > 0 && (!kinds || !eps || !sub)))
+        return fail(LT_EINVAL, "lt_integrator_schedule: bad arguments");
+    int i = 0;
+    auto put = [&](char k, double e, bool s) {
+        if (i < cap) {
+            kinds[i] = k;
+            eps[i] = e;
+            sub[i] = s ? 1 : 0;
+        }
+        ++i;
+    };
+    lt::metad_schedule(
+        integrator, tau, n_step,
+        [&](double e, bool s, int kick) { put(kick == lt::KICK_BOTH ? 'P' : kick == lt::KICK_GAUGE ? 'G' : 'M', e, s); },
+        [&](double e) { put('U', e, false); });
+    *n = i;
+    return LT_OK;
+}
+
+int lt_cv_path_samples(lt_cv cv, int deposited, double* out, int cap, int* n)
+{
+    if (!cv || !n || cap < 0 || (cap > 0 && !out)) return fail(LT_EINVAL, "lt_cv_path_samples: bad arguments");
+    const std::vector<double>& s = deposited ? cv->path_used : cv->path;
+    *n = (int)s.size();
+    std::copy_n(s.begin(), std::min<size_t>(s.size(), (size_t)cap), out);
+    return LT_OK;
+}
+
+int lt_cv_log_path(lt_cv cv, int deposited, const char* path)
+{
+    if (!cv || !path) return fail(LT_EINVAL, "lt_cv_log_path: bad arguments");
+    const std::vector<double>& s = deposited ? cv->path_used : cv->path;
+    if (s.empty()) return fail(LT_ESTATE, "lt_cv_log_path: no path samples (no MetaD-HMC trajectory yet)");
+    std::ofstream log(path, std::ios::out | std::ios::app);
+    if (!log) return fail(LT_EIO, std::string("lt_cv_log_path: cannot open ") + path);
+    for (size_t i = 0; i + 1 < s.size(); ++i) log << s[i] << ' ';
+    log << s.back() << '\n';
+    return log ? LT_OK : fail(LT_EIO, std::string("lt_cv_log_path: write failed: ") + path);
 }
 
 }  // extern "C"

@@ -191,7 +191,24 @@ inline unsigned map_blocks(const LinkMap& m)
 // momentum update, sub marks the updates whose CV the MetaD path update samples.
 void hmc_schedule(int kind, double tau, int n_step, const std::function<void(double, bool)>& P,
                   const std::function<void(double)>& U);
-inline bool valid_integrator(int k) { return k == LT_INT_LEAPFROG || k == LT_INT_OMF2 || k == LT_INT_OMF4; }
+inline bool valid_integrator(int k)
+{
+    return k == LT_INT_LEAPFROG || k == LT_INT_OMF2 || k == LT_INT_OMF4 || k == LT_INT_LEAPFROG_SLOW ||
+           k == LT_INT_OMF2_SLOW || k == LT_INT_OMF4_SLOW;
+}
+
+// Momentum kicks of a MetaD-HMC schedule: gauge force, bias force, or both in one update.
+enum Kick { KICK_GAUGE = 1, KICK_BIAS = 2, KICK_BOTH = 3 };
+// The MetaD-HMC sequence: the single-scale integrators kick with both forces
+// (HMCMetaDKernel::UpdateMomenta, hmc_metadynamics.hpp:441-459); the multiple-timescale
+// ones (hmc_metadynamics.hpp:28-238) kick the bias force on the outer scale and the
+// gauge force on the inner OMF4.  P(eps, sub, kick); sub marks the path samples.
+void metad_schedule(int kind, double tau, int n_step, const std::function<void(double, bool, int)>& P,
+                    const std::function<void(double)>& U);
+inline bool valid_metad_integrator(int k)
+{
+    return valid_integrator(k) || k == LT_INT_LEAPFROG_OMF4 || k == LT_INT_OMF2_OMF4_SLOW || k == LT_INT_OMF2_OMF4;
+}
 
 Geom make_geom(int nt, int nx, int ny, int nz);
 // tile map with extents (tt, tx, ty, tz) over (t, x, y, z/2); returns false if they do not divide
@@ -200,6 +217,8 @@ LinkMap default_linkmap(const Geom& g);
 
 // kernel launchers (defined in the .hip translation units)
 void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n);
+// y += a x over n complex entries (the bias-force kick P += eps F)
+void k_axpy(lt_ctx ctx, const double2* x, double2* y, double a, size_t n);
 void k_pack(lt_field f, const double* d_aos);
 void k_unpack(lt_field f, double* d_aos);
 void k_identity(lt_field f);

@@ -367,6 +367,123 @@ def metad_omf2(U, P, shape, beta, tau, n_step, rho, n_levels, bias):
     return H_old, H_new, cv_old, cv_new, samples
 
 
+OMF2_LAMBDA = 0.1931833275037836
+OMF4_ABGD = (0.08398315262876693, 0.2539785108410595, 0.6822365335719091, -0.03230286765269967)
+
+
+def integrator_schedule(name, tau, n_step):
+    """Kick / drift events [(kind, eps, sub)] of Integrators::HMC::* (hmc_gauge.hpp:21-295,
+    hmc_metadynamics.hpp:28-238): kind 'P' UpdateMomenta, 'G' UpdateGaugeMomenta,
+    'M' UpdateMetadynamicsMomenta, 'U' UpdateFields.  The reference records path samples
+    only in UpdateMomenta; for the multiple-timescale schedules sub marks the merged
+    and last bias kicks (the last one of each step for omf2_omf4_slow)."""
+    if n_step <= 0:
+        return []
+    e = tau / n_step
+    ev = []
+
+    def add(kind, x, sub=0):
+        ev.append((kind, x, sub))
+
+    a4, b4, g4, d4 = (c * e for c in OMF4_ABGD)
+    m4 = (0.5 - OMF4_ABGD[2] - OMF4_ABGD[0]) * e
+    n4 = (1.0 - 2.0 * OMF4_ABGD[3] - 2.0 * OMF4_ABGD[1]) * e
+
+    if name == "leapfrog_slow":
+        for _ in range(n_step):
+            add("P", 0.5 * e)
+            add("U", e)
+            add("P", 0.5 * e, 1)
+    elif name == "leapfrog":
+        add("P", 0.5 * e)
+        for _ in range(n_step - 1):
+            add("U", e)
+            add("P", e, 1)
+        add("U", e)
+        add("P", 0.5 * e, 1)
+    elif name in ("omf2", "omf2_slow"):
+        al, be, ga = OMF2_LAMBDA * e, 0.5 * e, (1.0 - 2.0 * OMF2_LAMBDA) * e
+        if name == "omf2_slow":
+            for _ in range(n_step):
+                ev.extend([("P", al, 0), ("U", be, 0), ("P", ga, 0), ("U", be, 0), ("P", al, 1)])
+        else:
+            ev.extend([("P", al, 0), ("U", be, 0), ("P", ga, 0), ("U", be, 0)])
+            for _ in range(n_step - 1):
+                ev.extend([("P", 2.0 * al, 1), ("U", be, 0), ("P", ga, 0), ("U", be, 0)])
+            add("P", al, 1)
+    elif name in ("omf4", "omf4_slow"):
+        body = [("U", b4, 0), ("P", g4, 0), ("U", d4, 0), ("P", m4, 0), ("U", n4, 0), ("P", m4, 0), ("U", d4, 0),
+                ("P", g4, 0), ("U", b4, 0)]
+        if name == "omf4_slow":
+            for _ in range(n_step):
+                ev.extend([("P", a4, 0)] + body + [("P", a4, 1)])
+        else:
+            ev.extend([("P", a4, 0)] + body)
+            for _ in range(n_step - 1):
+                ev.extend([("P", 2.0 * a4, 1)] + body)
+            add("P", a4, 1)
+    elif name in ("leapfrog_omf4", "omf2_omf4_slow", "omf2_omf4"):
+        h = 0.5 if name == "omf2_omf4" else 1.0
+        ca, cb, cg, cd = (h * c * e for c in OMF4_ABGD)
+        cm = h * (0.5 - OMF4_ABGD[2] - OMF4_ABGD[0]) * e
+        cn = h * (1.0 - 2.0 * OMF4_ABGD[3] - 2.0 * OMF4_ABGD[1]) * e
+        inner = [("G", ca, 0), ("U", cb, 0), ("G", cg, 0), ("U", cd, 0), ("G", cm, 0), ("U", cn, 0), ("G", cm, 0),
+                 ("U", cd, 0), ("G", cg, 0), ("U", cb, 0), ("G", ca, 0)]
+        if name == "leapfrog_omf4":
+            add("M", 0.5 * e)
+            for _ in range(n_step - 1):
+                ev.extend(inner)
+                add("M", e, 1)
+            ev.extend(inner)
+            add("M", 0.5 * e, 1)
+        else:
+            oa, ob = OMF2_LAMBDA * e, (1.0 - 2.0 * OMF2_LAMBDA) * e
+            if name == "omf2_omf4_slow":
+                for _ in range(n_step):
+                    ev.extend([("M", oa, 0)] + inner + [("M", ob, 0)] + inner + [("M", oa, 1)])
+            else:
+                add("M", oa)
+                for _ in range(n_step - 1):
+                    ev.extend(inner + [("M", ob, 0)] + inner + [("M", 2.0 * oa, 1)])
+                ev.extend(inner + [("M", ob, 0)] + inner + [("M", oa, 1)])
+    else:
+        raise ValueError(f"unknown integrator {name}")
+    return ev
+
+
+def metad_trajectory(U, P, shape, beta, tau, n_step, rho, n_levels, bias, integrator="omf2"):
+    """HMCMetaDKernel's molecular dynamics on given momenta with any integrator
+    (hmc_metadynamics.hpp:374-491): 'P' kicks add both forces, 'G' the gauge force,
+    'M' the bias force evaluated at the current field.  A zero bias (bias=None) gives
+    plain HMC.  Returns (H_old, H_new, cv_old, cv_new, samples); U, P evolve in place."""
+    samples = []
+
+    def charge():
+        return cv_charge(U, shape, rho, n_levels) if bias is not None else 0.0
+
+    def energy(q):
+        return hamiltonian(U, P, shape, beta) + (bias.potential(q) if bias is not None else 0.0)
+
+    cv_old = charge()
+    H_old = energy(cv_old)
+    for kind, e, sub in integrator_schedule(integrator, tau, n_step):
+        if kind == "U":
+            field_update(U, P, shape, e)
+            continue
+        F = None
+        if kind in ("P", "M") and bias is not None:
+            q = cv_charge(U, shape, rho, n_levels)
+            if sub:
+                samples.append(q)
+            _, F = cv_force(U, shape, rho, n_levels, bias.derivative(q))
+        if kind in ("P", "G"):
+            gauge_force_update(U, P, shape, e, beta)
+        if F is not None:
+            P[...] += e * F
+    cv_new = charge()
+    return H_old, energy(cv_new), cv_old, cv_new, samples
+
+
 def uniform(seed, site, stream, counter, k=0, sub=0, mu=0):
     """First uniform of the Philox draw with counter (site, stream<<24 | k<<4 | sub<<2 | mu, counter)
     and key seed: the stream-3 accept draw is uniform(seed, 0xFFFFFFFF, 3, counter)."""

@@ -191,6 +191,40 @@ struct InjectedOMF2MetaD
     }
 };
 
+// Stands in for the HMC functor of an integrator: records every momentum kick and
+// field drift with its step size, so the integrator's own operator() yields its
+// schedule (kind P = UpdateMomenta, G = UpdateGaugeMomenta, M = UpdateMetadynamicsMomenta,
+// U = UpdateFields; the flag is UpdateMomenta's submeasurement argument).
+struct ScheduleRecorder
+{
+    void UpdateMomenta(const floatT e, const bool sub = false) { std::printf("P %.17g %d\n", e, (int)sub); }
+    void UpdateGaugeMomenta(const floatT e) { std::printf("G %.17g 0\n", e); }
+    void UpdateMetadynamicsMomenta(const floatT e) { std::printf("M %.17g 0\n", e); }
+    void UpdateFields(const floatT e) { std::printf("U %.17g 0\n", e); }
+};
+
+template<typename IntegratorT>
+void print_schedule(const double tau, const int n_step)
+{
+    ScheduleRecorder rec;
+    IntegratorT{}(rec, tau, n_step);
+}
+
+// Friend integrator that only sets the kernel's cv_path_samples, so that the
+// kernel's own LogCVData writes them (hmc_metadynamics.hpp:537-545, 613).
+struct SetPathSamples
+{
+    static constexpr bool merged_momentum_updates = true;
+    static constexpr bool path_update_compatible  = true;
+    std::vector<double> values;
+
+    template<typename HMCFunctor>
+    void operator()(HMCFunctor& HMC, const double, const int)
+    {
+        HMC.cv_path_samples = values;
+    }
+};
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -466,6 +500,37 @@ int main(int argc, char** argv)
         B.UpdatePotentialSymmetric(deps);
         B.SaveParameters(argv[2], true);
         B.SavePotential(argv[2]);
+    } else if (cmd == "schedule") {              // schedule name tau n_step: kicks and drifts of an integrator
+        const std::string name = argv[2];
+        const double tau = std::atof(argv[3]);
+        const int n = std::atoi(argv[4]);
+        using namespace Integrators::HMC;
+        if (name == "Leapfrog") print_schedule<Leapfrog>(tau, n);
+        else if (name == "Leapfrog_slow") print_schedule<Leapfrog_slow>(tau, n);
+        else if (name == "OMF_2") print_schedule<OMF_2>(tau, n);
+        else if (name == "OMF_2_slow") print_schedule<OMF_2_slow>(tau, n);
+        else if (name == "OMF_4") print_schedule<OMF_4>(tau, n);
+        else if (name == "OMF_4_slow") print_schedule<OMF_4_slow>(tau, n);
+        else if (name == "Leapfrog_OMF_4") print_schedule<Leapfrog_OMF_4>(tau, n);
+        else if (name == "OMF_2_OMF_4_slow") print_schedule<OMF_2_OMF_4_slow>(tau, n);
+        else if (name == "OMF_2_OMF_4") print_schedule<OMF_2_OMF_4>(tau, n);
+        else { std::fprintf(stderr, "unknown integrator %s\n", name.c_str()); return 2; }
+    } else if (cmd == "cvlog") {                 // cvlog logfile v...: one LogCVData line of the values v
+        U->SetToIdentity();
+        MetaBiasPotential B(-8.0, 8.0, 800, 0.05, 0.02, 10.0, 1000.0);
+        auto Ucopy = std::make_unique<GaugeField>();
+        auto Mom = std::make_unique<GaugeField>();
+        pcg64 g(1);
+        PRNG4D<Nt, Nx, Ny, Nz, pcg64, floatT, int> prng(g);
+        RunStatistics stats;
+        std::ofstream devnull("/dev/null");
+        std::string logpath = argv[2];
+        GaugeUpdates::HMCMetaDData data(n_smear_meta);
+        SetPathSamples integ;
+        for (int i = 3; i < argc; ++i) integ.values.push_back(std::strtod(argv[i], nullptr));
+        auto act = GaugeAction::WilsonAction;
+        GaugeUpdates::HMCMetaDKernel HMC(*U, *Ucopy, *Mom, B, data, integ, act, prng, stats, devnull, logpath, 1.0, 0.12);
+        HMC(1, false);
     } else {
         std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
         return 2;

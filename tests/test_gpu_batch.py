@@ -79,10 +79,11 @@ def test_batch_replica_reproduces_golden_metad(la, lto, golden, gpu_ctx):
     b.close()
 
 
-def test_batch_matches_single_lattice_path(la, lto, gpu_ctx):
+@pytest.mark.parametrize("integrator", ["omf2", "leapfrog_omf4", "omf2_omf4"])
+def test_batch_matches_single_lattice_path(la, lto, gpu_ctx, integrator):
     """A batch of one at strength 1 and lt_metad_hmc_trajectory agree bit for bit while the
     bias is fixed; with deposits (device exp vs host libm exp) to rounding."""
-    flush("batch vs single-lattice trajectory")
+    flush(f"batch vs single-lattice trajectory {integrator}")
     U0 = smooth(lto, SMALL, 4)
     bias_a, bias_b = plain_bias(la), plain_bias(la)
     b = make_batch(la, gpu_ctx, [SMALL], [U0], [9], [1.0], bias_a)
@@ -91,12 +92,12 @@ def test_batch_matches_single_lattice_path(la, lto, gpu_ctx):
     U.set_rng(9, 0)
     Uc, P, F = la.GaugeField(gpu_ctx, SMALL), la.MatrixField(gpu_ctx, SMALL), la.MatrixField(gpu_ctx, SMALL)
     cv = la.CollectiveVariable(gpu_ctx, SMALL, 2, 0.12)
-    k = la.HMCMetaDKernel(U, Uc, P, F, bias_b, cv, "omf2", 6.0, 0.5, update_bias=False)
+    k = la.HMCMetaDKernel(U, Uc, P, F, bias_b, cv, integrator, 6.0, 0.5, update_bias=False)
     for step in range(3):
         update = step == 2
         k.update_bias = update
         acc = k(4, metropolis_step=True)
-        b.metad_hmc(4, beta=6.0, integrator="omf2", tau=0.5, metropolis=True, update_bias=update)
+        b.metad_hmc(4, beta=6.0, integrator=integrator, tau=0.5, metropolis=True, update_bias=update)
         res = b.results()
         assert res["dH"][0] == k.delta_H, f"step {step}: {res['dH'][0]!r} vs {k.delta_H!r}"
         assert bool(res["accepted"][0]) == acc

@@ -65,7 +65,7 @@ def test_force_and_field_updates(la, lto, gpu_ctx):
     assert abs(H - lto.hamiltonian(Ur, Pr, SMALL, 6.0)) <= 1e-12 * abs(H)
 
 
-@pytest.mark.parametrize("integrator", ["leapfrog", "omf2", "omf4"])
+@pytest.mark.parametrize("integrator", ["leapfrog", "omf2", "omf4", "leapfrog_slow", "omf2_slow", "omf4_slow"])
 def test_integrators_match_oracle(la, lto, gpu_ctx, integrator):
     flush(f"integrate {integrator}")
     U0 = smooth(lto, SMALL, 3)
@@ -73,11 +73,13 @@ def test_integrators_match_oracle(la, lto, gpu_ctx, integrator):
     U = gauge(la, gpu_ctx, SMALL, U0)
     P = matrix(la, gpu_ctx, SMALL, P0)
     la.Integrate(U, P, 6.0, integrator, 0.5, 6)
+    Ur, Pr = U0.copy(), P0.copy()
     if integrator == "omf2":
-        Ur, Pr = U0.copy(), P0.copy()
         lto.omf2(Ur, Pr, SMALL, 6.0, 0.5, 6)
-        assert rel_err(U.download(), Ur) <= 1e-10
-        assert rel_err(P.download(), Pr) <= 1e-10
+    else:   # the schedule-driven oracle with no bias force (schedules pinned to the reference's)
+        lto.metad_trajectory(Ur, Pr, SMALL, 6.0, 0.5, 6, 0.12, 1, None, integrator)
+    assert rel_err(U.download(), Ur) <= 1e-10
+    assert rel_err(P.download(), Pr) <= 1e-10
     # reversibility (hmc_gauge.hpp:585-599): flip momenta, integrate back
     Pm = -P.download()
     P.upload(Pm)
@@ -253,6 +255,65 @@ def test_metad_trajectory_vs_oracle_small(la, lto, golden, gpu_ctx):
     assert abs(k.delta_H - (H1 - H0)) <= 1e-8 * max(1.0, abs(H1 - H0))
     assert abs(k.cv_value - cv1) <= 1e-10
     assert rel_err(U.download(), Ur) <= 1e-10
+    cv.close()
+
+
+@pytest.mark.parametrize("integrator", ["leapfrog_omf4", "omf2_omf4", "omf2_omf4_slow", "omf4", "leapfrog",
+                                        "omf4_slow"])
+def test_metad_integrators_vs_oracle(la, lto, golden, gpu_ctx, integrator):
+    """Every MetaD-HMC schedule (the multiple-timescale ones of hmc_metadynamics.hpp:28-238
+    among them) against the oracle's trajectory on the same Philox momenta; the path
+    samples too."""
+    flush(f"metad {integrator} 4^4 vs oracle")
+    bias, obias = metad_setup(la, lto, golden)
+    U0 = smooth(lto, TINY, 8, 2)
+    U = gauge(la, gpu_ctx, TINY, U0)
+    Uc, P, F = gauge(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY)
+    cv = la.CollectiveVariable(gpu_ctx, TINY, 2, 0.12)
+    U.set_rng(3, 11)
+    k = la.HMCMetaDKernel(U, Uc, P, F, bias, cv, integrator, 6.0, 0.4, update_bias=False)
+    k(3, metropolis_step=False)
+    Ur = U0.copy()
+    Pr = lto.random_momentum(TINY, 3, 11)
+    H0, H1, _, cv1, samples = lto.metad_trajectory(Ur, Pr, TINY, 6.0, 0.4, 3, 0.12, 2, obias, integrator)
+    assert abs(k.delta_H - (H1 - H0)) <= 1e-8 * max(1.0, abs(H1 - H0))
+    assert abs(k.cv_value - cv1) <= 1e-10
+    assert rel_err(U.download(), Ur) <= 1e-10
+    got = k.path_samples()
+    assert len(got) == len(samples) == 3
+    assert np.max(np.abs(got - np.array(samples))) <= 1e-10
+    assert np.array_equal(k.path_samples(deposited=True), got)       # accepted: deposits = visited
+    cv.close()
+
+
+def test_hmc_log_lines(la, lto, golden, gpu_ctx, tmp_path):
+    """LogCVData after each trajectory (hmc_metadynamics.hpp:613,642): the visited CVs, then
+    with the Metropolis step the deposited ones (CV_old repeated after a rejection), in the
+    %g format the reference's stream writes."""
+    flush("hmc log lines")
+    bias, _ = metad_setup(la, lto, golden)
+    U = gauge(la, gpu_ctx, TINY, smooth(lto, TINY, 4, 2))
+    Uc, P, F = gauge(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY)
+    cv = la.CollectiveVariable(gpu_ctx, TINY, 2, 0.12)
+    U.set_rng(2, 0)
+    log = tmp_path / "hmc_log.txt"
+    k = la.HMCMetaDKernel(U, Uc, P, F, bias, cv, "omf2", 6.0, 0.5, update_bias=True, hmc_log_path=str(log))
+    expect = []
+    for step in range(4):
+        # coarse steps (n_step = 1) reject now and then; the deposited line then repeats CV_old
+        acc = k(1 if step % 2 else 6, metropolis_step=True)
+        visited = k.path_samples()
+        used = k.path_samples(deposited=True)
+        assert len(visited) == len(used) == (1 if step % 2 else 6)
+        if acc:
+            assert np.array_equal(used, visited)
+        else:
+            assert np.all(used == k.cv_value)      # CV_old, which the rejected trajectory keeps
+        expect.append(" ".join(format(v, "g") for v in visited))
+        expect.append(" ".join(format(v, "g") for v in used))
+    k(3, metropolis_step=False)
+    expect.append(" ".join(format(v, "g") for v in k.path_samples()))
+    assert log.read_text().splitlines() == expect
     cv.close()
 
 
