@@ -14,7 +14,7 @@ SHAPE = (16, 16, 16, 16)
 
 
 @pytest.fixture(scope="module")
-def configs(lto):
+def start_fields(lto):
     U0 = lto.haar(SHAPE, 0)
     Us = lto.stout(U0, SHAPE, 0.1, 3)
     return {"haar": U0, "smooth": Us}
@@ -38,23 +38,23 @@ def obs_close(a, b, tol):
     assert rel_err(a[7 + nt:], b[7 + nt:]) <= tol
 
 
-def test_staple(lto, golden, configs):
-    assert_field_matches(golden, "staple_haar", lto.staple_field(configs["haar"], SHAPE), 1e-13)
+def test_staple(lto, golden, start_fields):
+    assert_field_matches(golden, "staple_haar", lto.staple_field(start_fields["haar"], SHAPE), 1e-13)
 
 
 @pytest.mark.parametrize("schedule", [2, 4])
-def test_overrelaxation_one_sweep(lto, golden, configs, schedule):
-    W = configs["smooth"].copy()
+def test_overrelaxation_one_sweep(lto, golden, start_fields, schedule):
+    W = start_fields["smooth"].copy()
     lto.or_sweep(W, SHAPE, 1, schedule)
     assert_field_matches(golden, f"or{schedule}_smooth_n1", W, 1e-12)
     obs_close(lto.observables(W, SHAPE, 6.0), golden[f"or{schedule}_smooth_n1_obs"], 1e-12)
 
 
 @pytest.mark.parametrize("schedule", [2, 4])
-def test_overrelaxation_four_sweeps_observables(lto, golden, configs, schedule):
+def test_overrelaxation_four_sweeps_observables(lto, golden, start_fields, schedule):
     # link values drift at ~1e-11 after 4 sweeps (rounding amplified by the
     # SU(2) projections); the action is conserved and observables agree closely
-    W = configs["smooth"].copy()
+    W = start_fields["smooth"].copy()
     lto.or_sweep(W, SHAPE, 4, schedule)
     ref = golden[f"or{schedule}_smooth_n4_obs"]
     got = lto.observables(W, SHAPE, 6.0)
@@ -62,13 +62,13 @@ def test_overrelaxation_four_sweeps_observables(lto, golden, configs, schedule):
 
 
 @pytest.mark.parametrize("name", ["haar", "smooth"])
-def test_observables(lto, golden, configs, name):
-    obs_close(lto.observables(configs[name], SHAPE, 6.0), golden[f"obs_{name}"], 1e-12)
+def test_observables(lto, golden, start_fields, name):
+    obs_close(lto.observables(start_fields[name], SHAPE, 6.0), golden[f"obs_{name}"], 1e-12)
 
 
 @pytest.mark.parametrize("name", ["haar", "smooth"])
-def test_stout_levels(lto, golden, configs, name):
-    W = configs[name]
+def test_stout_levels(lto, golden, start_fields, name):
+    W = start_fields[name]
     for lev in range(4):
         W = lto.stout(W, SHAPE, 0.12, 1)
         obs_close(lto.observables(W, SHAPE, 6.0), golden[f"stout_{name}_obs"][lev], 1e-12)
@@ -76,26 +76,26 @@ def test_stout_levels(lto, golden, configs, name):
 
 
 @pytest.mark.parametrize("integrator", ["rk3", "euler"])
-def test_flow(lto, golden, configs, integrator):
-    W = lto.flow(configs["smooth"], SHAPE, 0.01, 10, integrator)
+def test_flow(lto, golden, start_fields, integrator):
+    W = lto.flow(start_fields["smooth"], SHAPE, 0.01, 10, integrator)
     assert_field_matches(golden, f"flow_{integrator}_smooth", W, 1e-12)
     obs_close(lto.observables(W, SHAPE, 6.0), golden[f"flow_{integrator}_smooth_obs"], 1e-12)
 
 
-def test_clover_derivative(lto, golden, configs):
-    assert_field_matches(golden, "clover_deriv_smooth", lto.clover_derivative(configs["smooth"], SHAPE), 1e-12)
+def test_clover_derivative(lto, golden, start_fields):
+    assert_field_matches(golden, "clover_deriv_smooth", lto.clover_derivative(start_fields["smooth"], SHAPE), 1e-12)
 
 
-def test_cv_force(lto, golden, configs):
-    Q, F = lto.cv_force(configs["smooth"], SHAPE, 0.12, 4, 1.0)
+def test_cv_force(lto, golden, start_fields):
+    Q, F = lto.cv_force(start_fields["smooth"], SHAPE, 0.12, 4, 1.0)
     assert abs(Q - golden["cv_force_smooth_Q"][0]) <= 1e-12 * max(1.0, abs(Q))
     assert_field_matches(golden, "cv_force_smooth", F, 1e-11)
 
 
-def test_cv_force_vs_reference_finite_differences(lto, golden, configs):
+def test_cv_force_vs_reference_finite_differences(lto, golden, start_fields):
     # the reference's FD (numerical_derivative.hpp:214, h = 1e-4) against the
     # oracle's analytic force, projected as Re tr(lambda_a i F) (:215)
-    _, F = lto.cv_force(configs["smooth"], SHAPE, 0.12, 4, 1.0)
+    _, F = lto.cv_force(start_fields["smooth"], SHAPE, 0.12, 4, 1.0)
     lam = gell_mann()
     flat = F.reshape(-1, 18)
     fd = golden["fd_h1e-4"]
@@ -107,8 +107,8 @@ def test_cv_force_vs_reference_finite_differences(lto, golden, configs):
         assert np.max(np.abs(an - fd[li])) <= 1e-6 * scale + 1e-12, (li, an, fd[li])
 
 
-def test_hmc_injected_momenta(lto, golden, configs):
-    U = configs["smooth"].copy()
+def test_hmc_injected_momenta(lto, golden, start_fields):
+    U = start_fields["smooth"].copy()
     P = lto.random_momentum(SHAPE, 5, 0)
     H0 = lto.hamiltonian(U, P, SHAPE, 6.0)
     lto.omf2(U, P, SHAPE, 6.0, 1.0, 10)
@@ -119,14 +119,14 @@ def test_hmc_injected_momenta(lto, golden, configs):
     assert_field_matches(golden, "hmc_omf2_smooth", U, 1e-9)
 
 
-def test_metad_trajectory_injected_momenta(lto, golden, configs):
+def test_metad_trajectory_injected_momenta(lto, golden, start_fields):
     """HMCMetaDKernel OMF2 trajectory (tau = 0.5, 5 steps, 4 stout levels, rho = 0.12) on injected
     momenta with a deposited well-tempered bias, against the reference."""
     a = golden["metad_bias_args"]
     bias = lto.Bias(a[0], a[1], int(a[2]), a[3], a[4], a[5], a[6], well_tempered=True)
     for d in golden["metad_bias_deposits"]:
         bias.deposit_symmetric(d)
-    U = configs["smooth"].copy()
+    U = start_fields["smooth"].copy()
     P = lto.random_momentum(SHAPE, 5, 0)
     H0, H1, cv0, cv1, samples = lto.metad_omf2(U, P, SHAPE, 6.0, 0.5, 5, 0.12, 4, bias)
     Href = golden["metad_omf2_smooth_H"]
