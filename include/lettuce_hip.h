@@ -291,6 +291,11 @@ int lt_bias_info(lt_bias b, double *params, int *bins, unsigned long long *excee
 int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F, lt_cv cv, lt_bias bias,
                             double beta, int integrator, double tau, int n_step, int metropolis,
                             int update_bias, double *dH, int *accepted, double *cv_new);
+/* CalculateClover<1> + CloverDerivative(U, Clover, site, mu) for every link
+ * (observables/clover.hpp:59-89, 238-321), times scale: F = scale * dQ_clov/dU as
+ * the reference's MetaD force uses it (hmc_metadynamics.hpp:420, scale = dV/dQ);
+ * F is a matrix field */
+int lt_clover_derivative(lt_field U, lt_field F, double scale);
 /* cv_path_samples of the last lt_metad_hmc_trajectory on cv (hmc_metadynamics.hpp:398-404):
  * deposited = 0 the CVs visited on the path, else the values deposited (CV_old repeated
  * after a rejection, hmc_metadynamics.hpp:632-638); *n = their number, at most cap copied */

@@ -36,7 +36,7 @@ __all__ = [
     "PhiloxState", "RectangularAction", "TopChargePlaquette", "WilsonLoop", "PolyakovLoop",
     "ParityUpdate", "MetadynamicsTemperingKernel", "LuescherWeiszAction", "IwasakiAction", "DBW2Action",
     "StapleAction", "IntegratorSchedule", "HMCKernel", "HMCMetaDKernel", "MetaBiasPotential",
-    "CollectiveVariable", "MatrixField", "ReplicaBatch",
+    "CollectiveVariable", "MatrixField", "ReplicaBatch", "CloverDerivative", "MetaDRun", "TemperingRun",
 ]
 
 CB2 = 2
@@ -464,6 +464,12 @@ class MatrixField(GaugeField):
         self._h = h
 
 
+def CloverDerivative(U, F, scale=1.0):
+    """CalculateClover<1> + CloverDerivative for every link (observables/clover.hpp:59-89,
+    238-321) into the matrix field F, times scale (dV/dQ in the MetaD force)."""
+    call("lt_clover_derivative", U.handle, F.handle, scale)
+
+
 def RandomMomentum(P, seed, counter=0):
     call("lt_hmc_random_momentum", P.handle, seed, counter)
 
@@ -828,3 +834,6 @@ class ReplicaBatch:
             self.close()
         except Exception:
             pass
+
+
+from .driver import MetaDRun, TemperingRun  # noqa: E402,F401

@@ -487,6 +487,27 @@ int lt_cv_create_checkpointed(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_
     });
 }
 
+int lt_clover_derivative(lt_field U, lt_field F, double scale)
+{
+    if (!U || !F || U == F || !U->su3 || !same_shape(U->g, F->g))
+        return fail(LT_EINVAL, "lt_clover_derivative: bad arguments");
+    return guarded([&] {
+        LT_HIP_CHECK(hipSetDevice(U->ctx->device));
+        double2* clover = alloc_buf((size_t)12 * lt::R_SLOTS * U->g.Vh);
+        try {
+            (void)lt::k_clover_charge(U->ctx, U->g, U->d, clover);
+            lt::k_clover_deriv(U->ctx, U->g, U->map, U->d, clover, F->d, scale);
+            LT_HIP_CHECK(hipStreamSynchronize(U->ctx->stream));
+        } catch (...) {
+            (void)hipFree(clover);
+            throw;
+        }
+        LT_HIP_CHECK(hipFree(clover));
+        F->su3 = false;
+        return LT_OK;
+    });
+}
+
 int lt_cv_create(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_levels, double rho, lt_cv* out)
 {
     return lt_cv_create_checkpointed(ctx, nt, nx, ny, nz, n_levels, rho, 0, out);

@@ -156,6 +156,24 @@ def test_cv_force_golden16(la, lto, golden, gpu_ctx):
     cv.close()
 
 
+def test_clover_derivative_golden16_and_oracle(la, lto, golden, gpu_ctx):
+    """lt_clover_derivative: the reference's CloverDerivative on every link at 16^4 (sampled
+    links and field sums), the oracle on other extents, linear in the scale."""
+    flush("clover derivative 16^4 and 4x6x8x10")
+    U0 = lto.stout(lto.haar(G16, 0), G16, 0.1, 3)
+    U = gauge(la, gpu_ctx, G16, U0)
+    F = matrix(la, gpu_ctx, G16)
+    la.CloverDerivative(U, F)
+    assert_field_matches(golden, "clover_deriv_smooth", F.download(), 1e-12)
+    shape = (4, 6, 8, 10)
+    V0 = smooth(lto, shape, 7, 2)
+    V, G = gauge(la, gpu_ctx, shape, V0), matrix(la, gpu_ctx, shape)
+    la.CloverDerivative(V, G, -2.5)
+    assert rel_err(G.download(), -2.5 * lto.clover_derivative(V0, shape)) <= 1e-12
+    with pytest.raises(la.LettuceError):
+        la.CloverDerivative(V, F)          # extents differ
+
+
 def test_cv_force_finite_differences_on_gpu(la, lto, gpu_ctx):
     """Central differences of the GPU charge against the GPU adjoint on random links:
     dQ/dh of U_l -> exp(i h/2 lambda_a) U_l = Re tr(lambda_a i F_l).  h = 1e-5 balances truncation
@@ -355,6 +373,100 @@ def test_checkpointed_metad_trajectory_equals_full(la, lto, golden, gpu_ctx):
     (d0, a0, q0, u0, g0), (d1, a1, q1, u1, g1) = out
     assert d0 == d1 and a0 == a1 and q0 == q1
     assert np.array_equal(u0, u1) and np.array_equal(g0, g1)
+
+
+def test_tempering_run_matches_oracle(la, lto, gpu_ctx):
+    """TemperingRun (SU3_main.cpp:600-616) against the oracle doing the same iteration:
+    tempering_nonmetadynamics_sweeps x (Checkerboard4 HB + OR) on the unbiased stream with
+    its Philox counters, a MetaD-HMC OMF2 trajectory with Metropolis (stream 3) and path
+    deposits on the biased stream, a swap proposal (stream 4) every second iteration."""
+    flush("tempering run vs oracle")
+    shape = (4, 4, 4, 8)
+    rho, nl, beta, tau, n_hmc = 0.12, 2, 6.0, 0.3, 3
+    A, B = smooth(lto, shape, 31, 2), smooth(lto, shape, 32, 2)
+    U, Ut = gauge(la, gpu_ctx, shape, A), gauge(la, gpu_ctx, shape, B)
+    U.set_rng(5, 0)
+    Ut.set_rng(6, 0)
+    Uc, P, F = gauge(la, gpu_ctx, shape), matrix(la, gpu_ctx, shape), matrix(la, gpu_ctx, shape)
+    bias = la.MetaBiasPotential(-8.0, 8.0, 800, 1.5, 0.3, 10.0, 1000.0, well_tempered=True)
+    ob = lto.Bias(-8.0, 8.0, 800, 1.5, 0.3, 10.0, 1000.0, well_tempered=True)
+    for d in (0.3, -0.5, 0.05):
+        bias.UpdatePotential(d)
+        ob.deposit(d)
+    cv = la.CollectiveVariable(gpu_ctx, shape, nl, rho)
+    hmc = la.HMCMetaDKernel(Ut, Uc, P, F, bias, cv, "omf2", beta, tau, update_bias=True)
+    temper = la.MetadynamicsTemperingKernel(U, Ut, bias, cv, seed=77)
+    run = la.TemperingRun(la.HeatbathKernel(U, beta), la.OverrelaxationSubgroupKernel(U), hmc, temper,
+                          n_heatbath=1, n_orelax=2, tempering_nonmetadynamics_sweeps=2, tempering_swap_period=2)
+    host = {"U": A.copy(), "Ut": B.copy()}
+    hb_counter, traj, swaps = 0, 0, 0
+    for n_count in range(4):
+        acc, swapped = run.step(n_count, n_hmc, True)
+        for _ in range(2):
+            hb_counter = lto.hb_sweep(host["U"], shape, beta, 1, 4, 5, hb_counter)
+            lto.or_sweep(host["U"], shape, 2, 4)
+        W = host["Ut"].copy()
+        H0, H1, q0, q1, samples = lto.metad_trajectory(W, lto.random_momentum(shape, 6, traj), shape, beta, tau,
+                                                      n_hmc, rho, nl, ob, "omf2")
+        expect_acc = lto.uniform(6, 0xFFFFFFFF, 3, traj) <= np.exp(-(H1 - H0))
+        traj += 1
+        assert acc == expect_acc, f"iteration {n_count}"
+        if acc:
+            host["Ut"], cur = W, q1
+        else:
+            samples, cur = [q0] * len(samples), q0
+        for s in samples:
+            ob.deposit_symmetric(s)
+        if n_count % 2 == 0:
+            q_new = lto.cv_charge(host["U"], shape, rho, nl)
+            expect_swap = lto.uniform(77, 0, 4, swaps) <= np.exp(-(ob.potential(q_new) - ob.potential(cur)))
+            swaps += 1
+            assert swapped == expect_swap, f"iteration {n_count}"
+            if swapped:
+                host["U"], host["Ut"] = host["Ut"], host["U"]
+                cur = q_new
+        else:
+            assert swapped is None
+        # rounding of the MetaD trajectory reaches a few 1e-10 per link entry and in the CV; a
+        # heatbath, Metropolis or swap decision that went the other way would differ at O(1)
+        assert abs(bias.ReturnCV_current() - cur) <= 1e-8
+        assert np.max(np.abs(U.download() - host["U"])) <= 1e-8, f"iteration {n_count}: unbiased stream"
+        assert np.max(np.abs(Ut.download() - host["Ut"])) <= 1e-8, f"iteration {n_count}: MetaD stream"
+        assert rel_err(Ut.download(), host["Ut"]) <= 1e-9
+        assert rel_err(bias.grid(), ob.grid) <= 1e-8     # deposits at CVs equal to ~1e-10
+        # the next iteration starts the oracle from the device's state: heatbath sweeps
+        # amplify rounding differences of the fields (3e-10 -> 1e-6 over two iterations)
+        host["U"], host["Ut"] = U.download(), Ut.download()
+        ob.grid = bias.grid()
+        cur = bias.ReturnCV_current()
+    cv.close()
+
+
+def test_metad_run_parity_coin(la, lto, golden, gpu_ctx):
+    """MetaDRun (SU3_main.cpp:523-531): a trajectory, then ParityUpdate on heads."""
+    flush("metad run parity")
+    bias, _ = metad_setup(la, lto, golden)
+    U = gauge(la, gpu_ctx, TINY, smooth(lto, TINY, 3, 2))
+    Uc, P, F = gauge(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY)
+    cv = la.CollectiveVariable(gpu_ctx, TINY, 2, 0.12)
+    U.set_rng(4, 0)
+    run = la.MetaDRun(la.HMCMetaDKernel(U, Uc, P, F, bias, cv, "omf2", 6.0, 0.3, update_bias=False), Uc,
+                      np.random.default_rng(3))
+    # the same trajectory without the parity step, from the same state each time
+    V = gauge(la, gpu_ctx, TINY)
+    plain = la.HMCMetaDKernel(V, gauge(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY),
+                              bias, cv, "omf2", 6.0, 0.3, update_bias=False)
+    coins = np.random.default_rng(3).integers(0, 2, size=6)
+    for k in range(6):
+        V.upload(U.download())
+        V.set_rng(*U.get_rng())
+        _, flipped = run.step(2, True)
+        plain(2, True)
+        assert flipped == bool(coins[k])
+        expect = lto.parity_update(V.download(), TINY) if flipped else V.download()
+        assert np.max(np.abs(U.download() - expect)) <= 1e-13, f"step {k}"
+    assert 0 < coins.sum() < 6
+    cv.close()
 
 
 def test_two_stream_tempering_swap(la, lto, gpu_ctx):
