@@ -4,6 +4,7 @@
 #pragma once
 
 #include <array>
+#include <complex>
 #include <cstdint>
 #include <filesystem>
 #include <fstream>
@@ -71,16 +72,43 @@ class GaugeField {
     std::array<int, 4> shape_;
 };
 
+// GaugeAction::Rectangular<2> (actions/gauge/rectangular_action.hpp:29-300): plaquette
+// and 1x2 rectangle coefficients; c_rect = 0 is the Wilson action
+struct RectangularAction {
+    double beta;
+    double c_plaq = 1.0, c_rect = 0.0;
+};
+inline RectangularAction WilsonGaugeAction(double beta) { return {beta, 1.0, 0.0}; }
+inline RectangularAction LuescherWeiszAction(double beta) { return {beta, 1.0 + 8.0 / 12.0, -1.0 / 12.0}; }
+inline RectangularAction IwasakiAction(double beta) { return {beta, 1.0 + 8.0 * 0.331, -0.331}; }
+inline RectangularAction DBW2Action(double beta) { return {beta, 1.0 + 8.0 * 1.4088, -1.4088}; }
+
 // HeatbathKernel / OverrelaxationSubgroupKernel (updates/heatbath.hpp:106-108,
-// updates/overrelaxation.hpp:87-89): whole sweeps, driven by Iterator below
+// updates/overrelaxation.hpp:87-89): whole sweeps, driven by Iterator below; with an
+// improved action (c_rect != 0) only Checkerboard4 applies
 struct HeatbathKernel {
     GaugeField& U;
     double beta;
-    void Sweep(int n, int schedule) const { LETTUCE_HIP_CALL(lt_heatbath_sweep(U.handle(), beta, n, schedule)); }
+    RectangularAction action{0.0, 1.0, 0.0};
+    void Sweep(int n, int schedule) const
+    {
+        if (action.c_rect == 0.0 && action.c_plaq == 1.0)
+            LETTUCE_HIP_CALL(lt_heatbath_sweep(U.handle(), beta, n, schedule));
+        else
+            LETTUCE_HIP_CALL(
+                lt_heatbath_sweep_action(U.handle(), beta, action.c_plaq, action.c_rect, n, schedule));
+    }
 };
 struct OverrelaxationSubgroupKernel {
     GaugeField& U;
-    void Sweep(int n, int schedule) const { LETTUCE_HIP_CALL(lt_or_sweep(U.handle(), n, schedule)); }
+    RectangularAction action{0.0, 1.0, 0.0};
+    void Sweep(int n, int schedule) const
+    {
+        if (action.c_rect == 0.0 && action.c_plaq == 1.0)
+            LETTUCE_HIP_CALL(lt_or_sweep(U.handle(), n, schedule));
+        else
+            LETTUCE_HIP_CALL(lt_or_sweep_action(U.handle(), action.c_plaq, action.c_rect, n, schedule));
+    }
 };
 
 // Iterator::Checkerboard / Checkerboard4 (iterators/iterators.hpp:65-169)
@@ -114,6 +142,50 @@ inline void StoutSmearing4D(const GaugeField& in, GaugeField& out, double rho = 
 {
     LETTUCE_HIP_CALL(lt_stout(in.handle(), out.handle(), rho, 1));
 }
+inline void StoutSmearingN(const GaugeField& in, GaugeField& out, int n_smear, double rho = 0.12)
+{
+    LETTUCE_HIP_CALL(lt_stout(in.handle(), out.handle(), rho, n_smear));
+}
+inline double WilsonActionValue(const GaugeField& U, double beta)
+{
+    double s = 0.0;
+    LETTUCE_HIP_CALL(lt_wilson_action(U.handle(), beta, &s));
+    return s;
+}
+// Rectangular<2>::ActionNormalized, e.g. the tree-level Symanzik action of Observables()
+inline double ActionNormalized(const GaugeField& U, const RectangularAction& a)
+{
+    double s = 0.0;
+    LETTUCE_HIP_CALL(lt_action_rectangular(U.handle(), a.c_plaq, a.c_rect, &s));
+    return s;
+}
+inline double TopChargePlaquette(const GaugeField& U)
+{
+    double q = 0.0;
+    LETTUCE_HIP_CALL(lt_topcharge_plaquette(U.handle(), &q));
+    return q;
+}
+// EnergyDensity::Plaquette / Clover / CloverTimeslice (field_strength_tensor.hpp:136-196)
+struct EnergyDensities {
+    double plaquette = 0.0, clover = 0.0;
+    std::vector<double> clover_timeslice;
+};
+inline EnergyDensities EnergyDensity(const GaugeField& U)
+{
+    EnergyDensities e;
+    e.clover_timeslice.resize(U.Shape()[0]);
+    LETTUCE_HIP_CALL(lt_energy(U.handle(), &e.plaquette, &e.clover, e.clover_timeslice.data()));
+    return e;
+}
+// PolyakovLoop (observables/polyakov_loop.hpp:31-49)
+inline std::complex<double> PolyakovLoop(const GaugeField& U)
+{
+    double re = 0.0, im = 0.0;
+    LETTUCE_HIP_CALL(lt_polyakov_loop(U.handle(), &re, &im));
+    return {re, im};
+}
+// ParityUpdate (updates/parity_update.hpp:22-47); U_copy is overwritten
+inline void ParityUpdate(GaugeField& U, GaugeField& U_copy) { LETTUCE_HIP_CALL(lt_parity_update(U.handle(), U_copy.handle())); }
 
 // GradientFlowKernel<Integrator, Rectangular<1>> (smearing/gradient_flow.hpp:252-367)
 class GradientFlowKernel {
@@ -146,6 +218,40 @@ class MatrixField {
   private:
     lt_field h_ = nullptr;
 };
+
+// WilsonLoop<n_start, n_end, reset>(U, U_chain) (observables/wilson_loop.hpp:63-128);
+// U_chain keeps the straight link products between calls
+inline double WilsonLoop(const GaugeField& U, MatrixField& U_chain, int n_start, int n_end, bool reset)
+{
+    double w = 0.0;
+    LETTUCE_HIP_CALL(lt_wilson_loop(U.handle(), U_chain.handle(), n_start, n_end, reset ? 1 : 0, &w));
+    return w;
+}
+// CalculateClover<1> + CloverDerivative on every link (observables/clover.hpp:59-89,238-321)
+inline void CloverDerivative(const GaugeField& U, MatrixField& F, double scale = 1.0)
+{
+    LETTUCE_HIP_CALL(lt_clover_derivative(U.handle(), F.handle(), scale));
+}
+
+// The kicks and drifts an integrator issues: kind 'P' (both forces), 'G' (gauge force),
+// 'M' (bias force) or 'U' (field update), step size, path-sample flag
+struct ScheduleEvent {
+    char kind;
+    double eps;
+    bool sub;
+};
+inline std::vector<ScheduleEvent> IntegratorSchedule(int integrator, double tau, int n_step)
+{
+    int n = 0;
+    LETTUCE_HIP_CALL(lt_integrator_schedule(integrator, tau, n_step, nullptr, nullptr, nullptr, 0, &n));
+    std::vector<char> k(n);
+    std::vector<double> e(n);
+    std::vector<int> s(n);
+    LETTUCE_HIP_CALL(lt_integrator_schedule(integrator, tau, n_step, k.data(), e.data(), s.data(), n, &n));
+    std::vector<ScheduleEvent> out;
+    for (int i = 0; i < n; ++i) out.push_back({k[i], e[i], s[i] != 0});
+    return out;
+}
 
 // HMCKernel(U, U_copy, P, Integrator, Action, prng, ..., trajectory_length)
 // (updates/hmc_gauge.hpp:381-600); momenta from the field's Philox stream
@@ -277,6 +383,34 @@ class HMCMetaDKernel {
     double beta_, tau_;
     bool update_bias_;
     std::string log_;
+};
+
+// MetadynamicsTemperingKernel(U, U_temper, ..., bias, prng, ...) (updates/tempering.hpp:24-73):
+// each call proposes the swap of the unbiased stream U and the MetaD stream U_temper,
+// with the uniform from Philox stream 4 at (seed, counter), and advances the counter
+class MetadynamicsTemperingKernel {
+  public:
+    MetadynamicsTemperingKernel(GaugeField& U, GaugeField& U_temper, MetaBiasPotential& bias, CollectiveVariable& cv,
+                                std::uint64_t seed, std::uint64_t counter = 0)
+        : U_(U), Ut_(U_temper), bias_(bias), cv_(cv), seed_(seed), counter_(counter)
+    {
+    }
+    bool operator()()
+    {
+        int acc = 0;
+        LETTUCE_HIP_CALL(lt_metad_tempering_swap(U_.handle(), Ut_.handle(), cv_.handle(), bias_.handle(), seed_,
+                                                 counter_, &acc, &delta_V));
+        ++counter_;
+        return acc != 0;
+    }
+    double delta_V = 0.0;
+
+  private:
+    GaugeField& U_;
+    GaugeField& Ut_;
+    MetaBiasPotential& bias_;
+    CollectiveVariable& cv_;
+    std::uint64_t seed_, counter_;
 };
 
 // Replica batch: the reference's two-stream MetadynamicsTemperingKernel

@@ -37,6 +37,28 @@ int main()
     std::printf("MetaD accepted %d dH %.6f labels %d %d\n", (int)accepted, HMC_MetaD.delta_H, Temper.Labels()[0],
                 Temper.Labels()[1]);
 
+    // the two-stream loop body of SU3_main.cpp:600-616 with the reference's own kernels
+    GaugeField U_temper(gpu, 8, 8, 8, 8);
+    U_temper.SetToHaar(3);
+    HMCMetaDKernel HMC_temper(U_temper, U_copy, P, F, Bias, CV, LT_INT_OMF2_OMF4, 6.0, 1.0, true, "hmc_log.txt");
+    MetadynamicsTemperingKernel ParallelTemperingSwap(U, U_temper, Bias, CV, 7);
+    HeatbathKernel HeatbathLW{U, 6.0, LuescherWeiszAction(6.0)};
+    OverrelaxationSubgroupKernel OverrelaxationLW{U, LuescherWeiszAction(6.0)};
+    for (int n_count = 0; n_count < 2; ++n_count) {
+        Iterator::Checkerboard4(HeatbathLW, 1);
+        Iterator::Checkerboard4(OverrelaxationLW, 4);
+        HMC_temper(4, true);
+        std::printf("swap %d\n", (int)ParallelTemperingSwap());
+    }
+    ParityUpdate(U, U_copy);
+    MatrixField Chain(gpu, 8, 8, 8, 8);
+    const EnergyDensities E = EnergyDensity(U);
+    std::printf("S_sym %.9f Q_plaq %.9f W2 %.9f W4 %.9f P %.9f E %.9f\n",
+                ActionNormalized(U, LuescherWeiszAction(6.0)), TopChargePlaquette(U), WilsonLoop(U, Chain, 0, 2, true),
+                WilsonLoop(U, Chain, 2, 4, false), std::abs(PolyakovLoop(U)), E.clover);
+    CloverDerivative(U, F, 1.0);
+    std::printf("%zu schedule events\n", IntegratorSchedule(LT_INT_LEAPFROG_OMF4, 1.0, 4).size());
+
     // rotating checkpoints and the bias file (SU3_main.cpp:287-294, 503-504, 631)
     CheckpointManager Checkpointer("checkpoints", 3, true);
     PhiloxState prng{U};
