@@ -349,6 +349,17 @@ def main():
     la.Iterator.Checkerboard4(lw_or, 2)
     ctx.synchronize()
     lw_or_ms = (time.perf_counter() - t0) / 2 * 1e3
+    # Wilson HB + 4 OR with the Checkerboard4 schedule the reference's main program uses
+    cb4_hb, cb4_or = la.HeatbathKernel(W, args.beta), la.OverrelaxationSubgroupKernel(W)
+    la.Iterator.Checkerboard4(cb4_hb, 1)
+    la.Iterator.Checkerboard4(cb4_or, 4)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        la.Iterator.Checkerboard4(cb4_hb, 1)
+        la.Iterator.Checkerboard4(cb4_or, 4)
+    ctx.synchronize()
+    cb4_update_ms = (time.perf_counter() - t0) / 3 * 1e3
     # BMW files written and read back through the device codec (#18BMW, all rows)
     io = {}
     try:
@@ -400,6 +411,7 @@ def main():
         "observables_pass_ms": obs_ms,
         "observables_rest_ms": obs_rest_ms,
         "lw_or_sweep_ms": lw_or_ms,
+        "cb4_update_ms": cb4_update_ms,
         "bmw_io": io,
         "plaquette": plaq,
         "thermalisation_s": t_therm,

@@ -93,9 +93,39 @@ __device__ __forceinline__ SU2 heatbath_su2(SU2 A, double prefactor, uint64_t se
     return su2_mul(X, Vd);
 }
 
-__device__ __forceinline__ bool cb4_member(const int c[4], int mu, int colour)
+// Site i in [0, V/4) of colour k of direction mu under Checkerboard4, where the colour
+// is (t + x + y + z + x_mu) mod 4 (iterators.hpp:141-169; extents divisible by 4):
+//   mu != z: every (t, x, y) row holds Nz/4 of them, every fourth z starting at
+//            (k - t - x - y - x_mu) mod 4;
+//   mu == z: the colour is (t + x + y + 2z) mod 4, so only rows with t + x + y = k mod 2
+//            hold any, Nz/2 each, every second z.
+// One thread per site of the class; consecutive threads walk z, then y, x, t.
+__device__ __forceinline__ bool cb4_site(const Geom& g, int mu, int k, int i, int c[4], int& s)
 {
-    return ((c[0] + c[1] + c[2] + c[3] + c[mu]) & 3) == colour;
+    if (i >= (g.V >> 2)) return false;
+    if (mu != 3) {
+        const int per_row = g.L[3] >> 2;
+        int row = i / per_row;
+        const int j = i - row * per_row;
+        c[2] = row % g.L[2];
+        row /= g.L[2];
+        c[1] = row % g.L[1];
+        c[0] = row / g.L[1];
+        c[3] = ((k - c[0] - c[1] - c[2] - c[mu]) & 3) + 4 * j;
+    } else {
+        const int per_row = g.L[3] >> 1;
+        int row = i / per_row;
+        const int j = i - row * per_row;
+        const int ny2 = g.L[2] >> 1;
+        const int yh = row % ny2;
+        row /= ny2;
+        c[1] = row % g.L[1];
+        c[0] = row / g.L[1];
+        c[2] = 2 * yh + ((k - c[0] - c[1]) & 1);
+        c[3] = (((k - c[0] - c[1] - c[2]) & 3) >> 1) + 2 * j;
+    }
+    s = ((c[0] * g.L[1] + c[1]) * g.L[2] + c[2]) * g.L[3] + c[3];
+    return true;
 }
 
 // overrelaxation with the software-pipelined staple (the heatbath's sampler
@@ -149,11 +179,8 @@ template <bool HEATBATH>
 __global__ void LT_LINK_KERNEL sweep_cb4_kernel(double2* __restrict__ U, Geom g, int mu, int colour,
                                                  double prefactor, uint64_t seed, uint64_t counter)
 {
-    int s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    if (s >= g.V) return;
-    int c[4];
-    coords_of(g, s, c);
-    if (!cb4_member(c, mu, colour)) return;
+    int c[4], s;
+    if (!cb4_site(g, mu, colour, xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x, c, s)) return;
     update_link<HEATBATH>(U, g, s, c, mu, parity_of(c), prefactor, seed, counter);
 }
 
@@ -187,11 +214,8 @@ __global__ void __launch_bounds__(BS) sweep_action_kernel(double2* __restrict__ 
                                                           double prefactor, uint64_t seed, uint64_t counter,
                                                           double c_plaq, double c_rect)
 {
-    int s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    if (s >= g.V) return;
-    int c[4];
-    coords_of(g, s, c);
-    if (!cb4_member(c, mu, colour)) return;
+    int c[4], s;
+    if (!cb4_site(g, mu, colour, xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x, c, s)) return;
     update_link_action<HEATBATH>(U, g, s, c, mu, parity_of(c), prefactor, seed, counter, c_plaq, c_rect);
 }
 
@@ -226,7 +250,7 @@ void sweeps(lt_field u, int n, int schedule, double prefactor, const char* name)
         for (int mu = 0; mu < 4; ++mu) {
             if (schedule == LT_CB4) {
                 for (int col = 0; col < 4; ++col)
-                    launch(ctx, name, sweep_cb4_kernel<HEATBATH>, dim3(nblocks(g.V, BS)), dim3(BS), 0, u->d, g, mu, col,
+                    launch(ctx, name, sweep_cb4_kernel<HEATBATH>, dim3(nblocks(g.V / 4, BS)), dim3(BS), 0, u->d, g, mu, col,
                            prefactor, u->seed, counter);
             } else {
                 const LinkMap& m = u->map;
@@ -247,7 +271,7 @@ void sweeps_action(lt_field u, int n, double prefactor, double c_plaq, double c_
         uint64_t counter = HEATBATH ? u->counter++ : 0;
         for (int mu = 0; mu < 4; ++mu)
             for (int col = 0; col < 4; ++col)
-                launch(ctx, name, sweep_action_kernel<HEATBATH>, dim3(nblocks(g.V, BS)), dim3(BS), 0, u->d, g, mu, col,
+                launch(ctx, name, sweep_action_kernel<HEATBATH>, dim3(nblocks(g.V / 4, BS)), dim3(BS), 0, u->d, g, mu, col,
                        prefactor, u->seed, counter, c_plaq, c_rect);
     }
 }
