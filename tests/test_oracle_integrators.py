@@ -154,3 +154,17 @@ def test_cv_log_format_equals_reference(sched):
     """LogCVData writes with the stream's default format: six significant digits, %g."""
     g = sched["cvlog"]
     assert " ".join(format(v, "g") for v in g["values"]) + "\n" == g["text"]
+
+
+def test_schedule_argument_checks():
+    """Unknown integrator codes and negative step counts are errors; zero steps is an
+    empty schedule."""
+    import ctypes
+
+    import lettuce_amd as la
+    assert la.IntegratorSchedule("omf4", 1.0, 0) == []
+    n = ctypes.c_int()
+    with pytest.raises(la.LettuceError):
+        la._lib.call("lt_integrator_schedule", 3, 1.0, 4, None, None, None, 0, ctypes.byref(n))
+    with pytest.raises(la.LettuceError):
+        la._lib.call("lt_integrator_schedule", la.OMF2, 1.0, -1, None, None, None, 0, ctypes.byref(n))
