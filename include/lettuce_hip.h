@@ -227,6 +227,11 @@ int lt_hmc_integrate(lt_field U, lt_field P, double beta, int integrator, double
  * (one counter value per trajectory), integrate, Metropolis (restores U on rejection) */
 int lt_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, double beta, int integrator, double tau,
                       int n_step, int metropolis, double *dH, int *accepted);
+/* HMCKernel::ReversibilityTest (hmc_gauge.hpp:585-599): momenta from U's Philox state,
+ * integrate, reverse the momenta, integrate again; H before and after (the reference
+ * prints them).  U and P are left at the end point. */
+int lt_hmc_reversibility_test(lt_field U, lt_field P, double beta, int integrator, double tau, int n_step,
+                              double *H_old, double *H_new);
 
 /* ---- collective variable and its adjoint -----------------------------------
  * Workspace of HMCMetaDData (hmc_metadynamics.hpp:251-260): n stout levels with
@@ -291,6 +296,12 @@ int lt_bias_info(lt_bias b, double *params, int *bins, unsigned long long *excee
 int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F, lt_cv cv, lt_bias bias,
                             double beta, int integrator, double tau, int n_step, int metropolis,
                             int update_bias, double *dH, int *accepted, double *cv_new);
+/* HMCMetaDKernel::ReversibilityTest (hmc_metadynamics.hpp:657-674): momenta from U's
+ * Philox state, integrate, reverse the momenta, integrate again; the biased H before and
+ * after (the reference prints them).  U and P are left at the end point, U close to its
+ * start for a reversible integrator.  F is force scratch. */
+int lt_metad_reversibility_test(lt_field U, lt_field P, lt_field F, lt_cv cv, lt_bias bias, double beta,
+                                int integrator, double tau, int n_step, double *H_old, double *H_new);
 /* CalculateClover<1> + CloverDerivative(U, Clover, site, mu) for every link
  * (observables/clover.hpp:59-89, 238-321), times scale: F = scale * dQ_clov/dU as
  * the reference's MetaD force uses it (hmc_metadynamics.hpp:420, scale = dV/dQ);

@@ -268,6 +268,14 @@ class HMCKernel {
                                            metropolis_step ? 1 : 0, &delta_H, &acc));
         return acc != 0;
     }
+    // integrate, reverse the momenta, integrate back; {H_old, H_new}
+    std::array<double, 2> ReversibilityTest(int n_step)
+    {
+        std::array<double, 2> h{};
+        LETTUCE_HIP_CALL(
+            lt_hmc_reversibility_test(U_.handle(), P_.handle(), beta_, integrator_, tau_, n_step, &h[0], &h[1]));
+        return h;
+    }
     double delta_H = 0.0;
 
   private:
@@ -369,6 +377,14 @@ class HMCMetaDKernel {
             if (metropolis_step) LETTUCE_HIP_CALL(lt_cv_log_path(cv_.handle(), 1, log_.c_str()));
         }
         return acc != 0;
+    }
+    // integrate with the bias, reverse the momenta, integrate back; {H_old, H_new}
+    std::array<double, 2> ReversibilityTest(int n_step)
+    {
+        std::array<double, 2> h{};
+        LETTUCE_HIP_CALL(lt_metad_reversibility_test(U_.handle(), P_.handle(), F_.handle(), cv_.handle(),
+                                                     bias_.handle(), beta_, integrator_, tau_, n_step, &h[0], &h[1]));
+        return h;
     }
     double delta_H = 0.0, cv_value = 0.0;
 

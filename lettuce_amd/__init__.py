@@ -511,6 +511,14 @@ class HMCKernel:
         self.delta_H = d.value
         return bool(a.value)
 
+    def ReversibilityTest(self, n_step):
+        """HMCKernel::ReversibilityTest (hmc_gauge.hpp:585-599): integrate, reverse the
+        momenta, integrate back; returns (H_old, H_new).  U is left at the end point."""
+        h0, h1 = ctypes.c_double(), ctypes.c_double()
+        call("lt_hmc_reversibility_test", self.U.handle, self.P.handle, self.beta, self.integrator,
+             self.trajectory_length, n_step, ctypes.byref(h0), ctypes.byref(h1))
+        return h0.value, h1.value
+
 
 class CollectiveVariable:
     """Workspace of the smeared clover charge and its adjoint (HMCMetaDData, hmc_metadynamics.hpp:251-260)."""
@@ -677,6 +685,16 @@ class HMCMetaDKernel:
             if metropolis_step:
                 self.LogCVData(True)
         return bool(a.value)
+
+    def ReversibilityTest(self, n_step):
+        """HMCMetaDKernel::ReversibilityTest (hmc_metadynamics.hpp:657-674): integrate with the
+        bias, reverse the momenta, integrate back; returns (H_old, H_new) with V(Q) included.
+        U is left at the end point; the bias is not updated."""
+        h0, h1 = ctypes.c_double(), ctypes.c_double()
+        call("lt_metad_reversibility_test", self.U.handle, self.P.handle, self.F.handle, self.cv.handle,
+             self.bias.handle, self.beta, self.integrator, self.trajectory_length, n_step, ctypes.byref(h0),
+             ctypes.byref(h1))
+        return h0.value, h1.value
 
     def path_samples(self, deposited=False):
         """cv_path_samples of the last trajectory: the visited CVs, or the deposited values."""

@@ -79,6 +79,9 @@ SIGNATURES = {
                                 I, ctypes.c_double, I, I, I, D, ctypes.POINTER(I), D],
     "lt_cv_path_samples": [ctypes.c_void_p, I, D, I, ctypes.POINTER(I)],
     "lt_clover_derivative": [c_field, c_field, ctypes.c_double],
+    "lt_hmc_reversibility_test": [c_field, c_field, ctypes.c_double, I, ctypes.c_double, I, D, D],
+    "lt_metad_reversibility_test": [c_field, c_field, c_field, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, I,
+                                    ctypes.c_double, I, D, D],
     "lt_integrator_schedule": [I, ctypes.c_double, I, ctypes.c_char_p, D, ctypes.POINTER(I), I, ctypes.POINTER(I)],
     "lt_cv_log_path": [ctypes.c_void_p, I, ctypes.c_char_p],
     "lt_batch_create": [c_ctx, I, ctypes.POINTER(I), I, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)],

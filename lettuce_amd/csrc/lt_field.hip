@@ -165,7 +165,26 @@ __global__ void __launch_bounds__(BS) axpy_kernel(const double2* __restrict__ x,
     }
 }
 
+__global__ void __launch_bounds__(BS) scale_kernel(double2* __restrict__ y, double a, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * BS;
+    for (; i < n; i += stride) {
+        double2 v = y[i];
+        v.x *= a;
+        v.y *= a;
+        y[i] = v;
+    }
+}
+
 }  // namespace
+
+void k_scale(lt_ctx ctx, double2* y, double a, size_t n)
+{
+    unsigned nb = nblocks((long long)n, BS);
+    if (nb > 65536u) nb = 65536u;
+    launch(ctx, "scale_links", scale_kernel, dim3(nb), dim3(BS), 0, y, a, n);
+}
 
 void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n)
 {

@@ -350,6 +350,30 @@ double cv_charge(lt_cv cv, lt_field U)
     return cv_forward(cv, U);
 }
 
+// The molecular dynamics of HMCMetaDKernel (Integrator(*this, ...), hmc_metadynamics.hpp:374-524)
+// on the momenta in P; F is force scratch; the CVs of the sampling kicks are appended to samples.
+void metad_integrate(lt_field U, lt_field P, lt_field F, lt_cv cv, lt_bias bias, double beta, int integrator,
+                     double tau, int n_step, std::vector<double>& samples)
+{
+    lt::metad_schedule(
+        integrator, tau, n_step,
+        [&](double e, bool sub, int kick) {
+            if (kick & lt::KICK_BIAS) {
+                // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436)
+                const double q = cv_forward(cv, U);
+                if (sub) samples.push_back(q);
+                cv_backward(cv, bias->derivative(q), F->d);
+            }
+            if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)
+                lt::k_force_update(U, P, F->d, e * beta / 6.0, e);
+            else if (kick == lt::KICK_GAUGE)   // UpdateGaugeMomenta (:464-477)
+                lt::k_force_update(U, P, nullptr, e * beta / 6.0, 0.0);
+            else   // UpdateMetadynamicsMomenta (:479-491)
+                lt::k_axpy(U->ctx, F->d, P->d, e, (size_t)72 * P->g.Vh);
+        },
+        [&](double e) { lt::k_field_update(U, P, e); });
+}
+
 }  // namespace
 
 extern "C" {
@@ -436,6 +460,30 @@ int lt_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, double beta, int 
         }
         if (dH) *dH = d;
         if (accepted) *accepted = acc;
+        return LT_OK;
+    });
+}
+
+int lt_hmc_reversibility_test(lt_field U, lt_field P, double beta, int integrator, double tau, int n_step,
+                              double* H_old, double* H_new)
+{
+    if (!U || !P || U == P || !same_shape(U->g, P->g) || !valid_integrator(integrator) || n_step < 0)
+        return fail(LT_EINVAL, "lt_hmc_reversibility_test: bad arguments");
+    return guarded([&] {
+        // HMCKernel::ReversibilityTest (hmc_gauge.hpp:585-599)
+        lt::k_random_momentum(P, U->seed, U->counter++);
+        P->su3 = false;
+        const double H0 = hamiltonian(U, P, beta);
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass) lt::k_scale(U->ctx, P->d, -1.0, (size_t)72 * P->g.Vh);   // ReverseMomenta
+            integrate(
+                integrator, tau, n_step,
+                [&](double e, bool) { lt::k_force_update(U, P, nullptr, e * beta / 6.0, 0.0); },
+                [&](double e) { lt::k_field_update(U, P, e); });
+        }
+        const double H1 = hamiltonian(U, P, beta);
+        if (H_old) *H_old = H0;
+        if (H_new) *H_new = H1;
         return LT_OK;
     });
 }
@@ -675,23 +723,7 @@ int lt_metad_hmc_trajectory(lt_field U, lt_field P, lt_field U_copy, lt_field F,
         const double H0 = hamiltonian(U, P, beta) + bias->potential(cv_old);
         std::vector<double> samples;
         samples.reserve(n_step);
-        lt::metad_schedule(
-            integrator, tau, n_step,
-            [&](double e, bool sub, int kick) {
-                if (kick & lt::KICK_BIAS) {
-                    // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436)
-                    const double q = cv_forward(cv, U);
-                    if (sub) samples.push_back(q);
-                    cv_backward(cv, bias->derivative(q), F->d);
-                }
-                if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)
-                    lt::k_force_update(U, P, F->d, e * beta / 6.0, e);
-                else if (kick == lt::KICK_GAUGE)   // UpdateGaugeMomenta (:464-477)
-                    lt::k_force_update(U, P, nullptr, e * beta / 6.0, 0.0);
-                else   // UpdateMetadynamicsMomenta (:479-491)
-                    lt::k_axpy(U->ctx, F->d, P->d, e, (size_t)72 * P->g.Vh);
-            },
-            [&](double e) { lt::k_field_update(U, P, e); });
+        metad_integrate(U, P, F, cv, bias, beta, integrator, tau, n_step, samples);
         cv->path = samples;
         const double cv1 = cv_charge(cv, U);
         const double H1 = hamiltonian(U, P, beta) + bias->potential(cv1);
@@ -742,6 +774,29 @@ int lt_integrator_schedule(int integrator, double tau, int n_step, char* kinds, 
         [&](double e) { put('U', e, false); });
     *n = i;
     return LT_OK;
+}
+
+int lt_metad_reversibility_test(lt_field U, lt_field P, lt_field F, lt_cv cv, lt_bias bias, double beta,
+                                int integrator, double tau, int n_step, double* H_old, double* H_new)
+{
+    if (!U || !P || !F || !cv || !bias || !lt::valid_metad_integrator(integrator) || n_step < 0 || U == P || U == F ||
+        P == F || !same_shape(U->g, P->g) || !same_shape(U->g, F->g) || !same_shape(U->g, cv->g))
+        return fail(LT_EINVAL, "lt_metad_reversibility_test: bad arguments");
+    return guarded([&] {
+        // HMCMetaDKernel::ReversibilityTest (hmc_metadynamics.hpp:657-674)
+        lt::k_random_momentum(P, U->seed, U->counter++);
+        P->su3 = false;
+        F->su3 = false;
+        const double H0 = hamiltonian(U, P, beta) + bias->potential(cv_charge(cv, U));
+        std::vector<double> samples;
+        metad_integrate(U, P, F, cv, bias, beta, integrator, tau, n_step, samples);
+        lt::k_scale(U->ctx, P->d, -1.0, (size_t)72 * P->g.Vh);   // ReverseMomenta
+        metad_integrate(U, P, F, cv, bias, beta, integrator, tau, n_step, samples);
+        const double H1 = hamiltonian(U, P, beta) + bias->potential(cv_charge(cv, U));
+        if (H_old) *H_old = H0;
+        if (H_new) *H_new = H1;
+        return LT_OK;
+    });
 }
 
 int lt_cv_path_samples(lt_cv cv, int deposited, double* out, int cap, int* n)

@@ -219,6 +219,8 @@ LinkMap default_linkmap(const Geom& g);
 void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n);
 // y += a x over n complex entries (the bias-force kick P += eps F)
 void k_axpy(lt_ctx ctx, const double2* x, double2* y, double a, size_t n);
+// y *= a over n complex entries (ReverseMomenta with a = -1)
+void k_scale(lt_ctx ctx, double2* y, double a, size_t n);
 void k_pack(lt_field f, const double* d_aos);
 void k_unpack(lt_field f, double* d_aos);
 void k_identity(lt_field f);

@@ -304,6 +304,29 @@ def test_metad_integrators_vs_oracle(la, lto, golden, gpu_ctx, integrator):
     cv.close()
 
 
+@pytest.mark.parametrize("integrator", ["omf2", "omf4", "leapfrog_omf4", "omf2_omf4", "omf2_omf4_slow"])
+def test_reversibility_tests(la, lto, golden, gpu_ctx, integrator):
+    """ReversibilityTest (hmc_gauge.hpp:585-599, hmc_metadynamics.hpp:657-674): forward,
+    reversed momenta, back to the start -- the field to rounding and H_new = H_old."""
+    flush(f"reversibility {integrator}")
+    bias, _ = metad_setup(la, lto, golden)
+    U0 = smooth(lto, TINY, 12, 2)
+    U = gauge(la, gpu_ctx, TINY, U0)
+    Uc, P, F = gauge(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY), matrix(la, gpu_ctx, TINY)
+    cv = la.CollectiveVariable(gpu_ctx, TINY, 2, 0.12)
+    U.set_rng(8, 0)
+    h0, h1 = la.HMCMetaDKernel(U, Uc, P, F, bias, cv, integrator, 6.0, 0.5).ReversibilityTest(4)
+    assert abs(h1 - h0) <= 1e-9 * abs(h0)
+    assert np.max(np.abs(U.download() - U0)) <= 1e-10
+    assert U.get_rng() == (8, 1)
+    if integrator in ("omf2", "omf4"):
+        U.upload(U0)
+        g0, g1 = la.HMCKernel(U, Uc, P, integrator, 6.0, 0.5).ReversibilityTest(4)
+        assert abs(g1 - g0) <= 1e-9 * abs(g0)
+        assert np.max(np.abs(U.download() - U0)) <= 1e-10
+    cv.close()
+
+
 def test_hmc_log_lines(la, lto, golden, gpu_ctx, tmp_path):
     """LogCVData after each trajectory (hmc_metadynamics.hpp:613,642): the visited CVs, then
     with the Metropolis step the deposited ones (CV_old repeated after a rejection), in the
