@@ -23,6 +23,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size L] [--therm N
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -69,6 +70,37 @@ def cpu_baseline(L, beta, budget_s):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": links / t_tot, "unit": "link updates/s", "cores": threads, "kind": "port",
             "sample": f"{n_upd} x (1 HB + 4 OR) sweeps of {L}^4 by the OpenMP C oracle, {t_tot:.1f} s"}
+
+
+def reference_cpu_16(beta, nrep=3):
+    """The reference's own OpenMP code (oracle/_ref/lettuce_ref, built from its headers;
+    extents fixed at 16^4 when it is compiled): seconds per HB sweep, OR sweep and RK3
+    step on a Haar start.  None when the harness was not built."""
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "lettuce_ref")
+    if not os.access(exe, os.X_OK):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lto
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "u.bin")
+        lto.haar((16, 16, 16, 16), 0).tofile(path)
+        r = subprocess.run([exe, "time", path, repr(beta), str(nrep)], capture_output=True, text=True, timeout=300)
+    if r.returncode:
+        return None
+    kv = {}
+    for line in r.stdout.splitlines():
+        parts = line.split()
+        if len(parts) == 2 and parts[0] in ("hb_sweep_s", "or_sweep_s", "rk3_step_s", "threads"):
+            kv[parts[0]] = float(parts[1])
+    if len(kv) < 4:
+        return None
+    links = 4 * 16 ** 4
+    return {"lattice": [16, 16, 16, 16], "threads": int(kv["threads"]),
+            "hb_or_link_updates_per_s": 5 * links / (kv["hb_sweep_s"] + 4 * kv["or_sweep_s"]),
+            "hb_sweep_ms": kv["hb_sweep_s"] * 1e3, "or_sweep_ms": kv["or_sweep_s"] * 1e3,
+            "rk3_step_ms": kv["rk3_step_s"] * 1e3,
+            "note": "the reference code itself (checkerboard-2 sweeps, RK3 Wilson flow), timed on the host cores"}
 
 
 def thermalised(la, ctx, shape, seed, n_update, beta=6.0):
@@ -248,6 +280,14 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    # the reference's own code runs as a child process before this process touches the GPU
+    ref_cpu = None
+    if not args.no_cpu:
+        try:
+            ref_cpu = reference_cpu_16(args.beta)
+        except (OSError, ValueError, subprocess.SubprocessError):
+            ref_cpu = None
 
     import lettuce_amd as la
 
@@ -436,6 +476,7 @@ def main():
         result["cpu_baseline"] = cpu_baseline(L, args.beta, args.cpu_seconds)
     else:
         result["cpu_baseline"] = None
+    result["reference_cpu_16x4"] = ref_cpu
     print(json.dumps(result))
 
 
