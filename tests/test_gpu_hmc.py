@@ -339,6 +339,9 @@ def test_hmc_log_lines(la, lto, golden, gpu_ctx, tmp_path):
     U.set_rng(2, 0)
     log = tmp_path / "hmc_log.txt"
     k = la.HMCMetaDKernel(U, Uc, P, F, bias, cv, "omf2", 6.0, 0.5, update_bias=True, hmc_log_path=str(log))
+    with pytest.raises(la.LettuceError):
+        k.LogCVData()                      # no trajectory yet: nothing to log
+    assert len(k.path_samples()) == 0
     expect = []
     for step in range(4):
         # coarse steps (n_step = 1) reject now and then; the deposited line then repeats CV_old
@@ -355,6 +358,9 @@ def test_hmc_log_lines(la, lto, golden, gpu_ctx, tmp_path):
     k(3, metropolis_step=False)
     expect.append(" ".join(format(v, "g") for v in k.path_samples()))
     assert log.read_text().splitlines() == expect
+    k.hmc_log_path = str(tmp_path / "missing_dir" / "hmc_log.txt")
+    with pytest.raises(la.LettuceError):
+        k.LogCVData()                      # unwritable path: an error, not a silent skip
     cv.close()
 
 
