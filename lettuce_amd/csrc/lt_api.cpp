@@ -51,6 +51,12 @@ bool make_linkmap(const Geom& g, int tt, int tx, int ty, int tz, LinkMap& m)
 #if LT_TCHUNK
     if (m.N[0] % 8 == 0) m.tc = m.N[0] / 8;
 #endif
+    // y chunks: the tiles in flight per x step (tc * yc of them) and the x-neighbour
+    // tiles just finished share the XCD's L2
+    m.yc = m.N[2];
+#if LT_YCHUNK > 0
+    if (m.N[2] % LT_YCHUNK == 0) m.yc = LT_YCHUNK;
+#endif
     return true;
 }
 

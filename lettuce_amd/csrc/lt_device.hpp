@@ -25,6 +25,12 @@
 #ifndef LT_TCHUNK
 #define LT_TCHUNK 1
 #endif
+// y tiles per chunk (LinkMap::yc; 0: all y tiles in one chunk).  Two y tiles per
+// chunk at 32^4 (8 tiles in flight per x step in each XCD): OR sweep -2 %, RK3 flow
+// step -1 % against one chunk; one y tile per chunk is slower.
+#ifndef LT_YCHUNK
+#define LT_YCHUNK 2
+#endif
 #define LT_LINK_KERNEL __launch_bounds__(LT_LINK_BS) __attribute__((amdgpu_waves_per_eu(LT_LINK_WAVES)))
 
 namespace lt {
@@ -897,8 +903,10 @@ struct LinkMap {
     int N[4];     // number of tiles in t, x, y, zc
     int nthreads; // T[0]*T[1]*T[2]*T[3] == blockDim.x
     int tc;       // t tiles per chunk (divides N[0])
+    int yc;       // y tiles per chunk (divides N[2])
 };
 
+// tile order, fastest first: t within its chunk, z, y within its chunk, x, y chunk, t chunk
 __device__ __forceinline__ bool map_site_b(const Geom& g, const LinkMap& m, int b, int par, int c[4], int& s)
 {
     int tid = threadIdx.x;
@@ -906,10 +914,13 @@ __device__ __forceinline__ bool map_site_b(const Geom& g, const LinkMap& m, int 
     b /= m.tc;
     int bz = b % m.N[3];
     b /= m.N[3];
-    int by = b % m.N[2];
-    b /= m.N[2];
+    const int by_lo = b % m.yc;
+    b /= m.yc;
     int bx = b % m.N[1];
-    int bt = (b / m.N[1]) * m.tc + bt_lo;
+    b /= m.N[1];
+    const int ny_hi = m.N[2] / m.yc;
+    int by = (b % ny_hi) * m.yc + by_lo;
+    int bt = (b / ny_hi) * m.tc + bt_lo;
     if (bt >= m.N[0] || tid >= m.nthreads) return false;
     int dz = tid % m.T[3];
     tid /= m.T[3];

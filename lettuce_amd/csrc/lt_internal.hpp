@@ -181,7 +181,7 @@ inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / 
 inline unsigned map_blocks(const LinkMap& m)
 {
     if (m.nthreads < 1 || m.nthreads > LT_LINK_BS || m.T[0] * m.T[1] * m.T[2] * m.T[3] != m.nthreads || m.tc < 1
-        || m.N[0] % m.tc)
+        || m.N[0] % m.tc || m.yc < 1 || m.N[2] % m.yc)
         throw LtError{LT_ESTATE, "invalid thread-block tile for the link kernels"};
     return (unsigned)(m.N[0] * m.N[1] * m.N[2] * m.N[3]);
 }
