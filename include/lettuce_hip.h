@@ -157,6 +157,12 @@ int lt_polyakov_loop(lt_field u, double *re, double *im);
 /* TopChargeClover (topological_charge.hpp:97-231); q_t (Nt entries, may be NULL)
  * as TopChargeCloverTimeslice (:337-364) */
 int lt_topcharge_clover(lt_field u, double *Q, double *q_t);
+/* The per-site summand of TopChargeClover (topological_charge.hpp:184-199):
+ * q_x[s] = -1/(256 pi^2) Re tr(F01 F23 - F02 F13 + F03 F12), F = C - C^+, for the V
+ * sites in the reference's lexicographic order ((t*Nx+x)*Ny+y)*Nz+z.  Sums of
+ * differences of two such densities are free of the rounding of a global sum
+ * (finite-difference checks of the charge's derivative). */
+int lt_topcharge_clover_density(lt_field u, double *q_x);
 /* EnergyDensity::Clover / CloverTimeslice on traceless F (field_strength_tensor.hpp:
  * 107-134,154-195) and EnergyDensity::Plaquette (:140-145); e_t may be NULL */
 int lt_energy(lt_field u, double *e_plaq, double *e_clov, double *e_t);

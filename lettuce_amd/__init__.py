@@ -29,8 +29,8 @@ from .io import (CheckpointManager, LoadConfigBMW, LoadConfigBMWFull, PhiloxStat
                  SaveConfigBMWFull)
 
 __all__ = [
-    "Context", "GaugeField", "Iterator", "HeatbathKernel", "OverrelaxationSubgroupKernel",
-    "PlaquetteSum", "TopChargeClover", "TopChargeCloverTimeslice", "EnergyDensity", "WilsonAction",
+    "Context", "GaugeField", "Iterator", "HeatbathKernel", "OverrelaxationSubgroupKernel", "PlaquetteSum",
+    "TopChargeClover", "TopChargeCloverTimeslice", "TopChargeCloverDensity", "EnergyDensity", "WilsonAction",
     "StoutSmearing4D", "StoutSmearingN", "GradientFlowKernel", "Observables", "LettuceError",
     "CB2", "CB4", "SaveConfigBMW", "SaveConfigBMWFull", "LoadConfigBMW", "LoadConfigBMWFull", "CheckpointManager",
     "PhiloxState", "RectangularAction", "TopChargePlaquette", "WilsonLoop", "PolyakovLoop",
@@ -282,6 +282,13 @@ def TopChargeClover(U):
     v = ctypes.c_double()
     call("lt_topcharge_clover", U.handle, ctypes.byref(v), None)
     return v.value
+
+
+def TopChargeCloverDensity(U):
+    """Per-site summand of TopChargeClover (topological_charge.hpp:184-199), shape (Nt, Nx, Ny, Nz)."""
+    q = np.zeros(U.shape)
+    call("lt_topcharge_clover_density", U.handle, _dp(q))
+    return q
 
 
 def TopChargeCloverTimeslice(U):

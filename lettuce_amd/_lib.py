@@ -45,6 +45,7 @@ SIGNATURES = {
     "lt_plaquette_sum": [c_field, D],
     "lt_wilson_action": [c_field, ctypes.c_double, D],
     "lt_topcharge_clover": [c_field, D, D],
+    "lt_topcharge_clover_density": [c_field, D],
     "lt_energy": [c_field, D, D, D],
     "lt_observables": [c_field, D],
     "lt_stout": [c_field, c_field, ctypes.c_double, I],

@@ -700,6 +700,17 @@ int lt_topcharge_clover(lt_field u, double* Q, double* q_t)
     return LT_OK;
 }
 
+int lt_topcharge_clover_density(lt_field u, double* q_x)
+{
+    if (!u || !q_x) return fail(LT_EINVAL, "null argument");
+    if (!u->su3) return fail(LT_ESTATE, "lt_topcharge_clover_density: the field must hold SU(3) links");
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_topcharge_density(u, q_x);
+        return LT_OK;
+    });
+}
+
 int lt_energy(lt_field u, double* e_plaq, double* e_clov, double* e_t)
 {
     if (!u) return fail(LT_EINVAL, "null argument");

@@ -241,6 +241,8 @@ void k_staple_field_action(lt_field u, lt_field out, double c_plaq, double c_rec
 void k_or_sweep_action(lt_field u, int n, double c_plaq, double c_rect);
 void k_heatbath_sweep_action(lt_field u, double beta, int n, double c_plaq, double c_rect);
 void k_observables(lt_field u, double* host_out);
+// per-site clover charge density (V doubles, lexicographic order) into host memory
+void k_topcharge_density(lt_field u, double* host_q);
 void k_rect_sums(lt_field u, double out[3]);
 double k_wilson_loop(lt_field u, lt_field chain, int n0, int n1, int reset);
 void k_polyakov_loop(lt_field u, double* re, double* im);

@@ -105,6 +105,23 @@ __global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, 
     block_sum3(p, q, e, part + 3 * (((size_t)pair * g.L[0] + t) * nchunk + chunk));
 }
 
+// TopChargeClover's summand per site (topological_charge.hpp:184-199): one thread
+// per site (lexicographic order) evaluates all three plane pairs and writes
+// -1/(256 pi^2) (tr F01F23 - tr F02F13 + tr F03F12), F = C - C^+.
+__global__ void __launch_bounds__(BS) qdensity_kernel(const double2* __restrict__ U, Geom g, double* __restrict__ q_x)
+{
+    const int s = blockIdx.x * BS + threadIdx.x;
+    if (s >= g.V) return;
+    int c[4];
+    coords_of(g, s, c);
+    const int par = parity_of(c);
+    double p = 0.0, e = 0.0, q01 = 0.0, q02 = 0.0, q03 = 0.0;
+    pair_terms<0, 1, 2, 3>(U, g, s, c, par, p, q01, e);
+    pair_terms<0, 2, 1, 3>(U, g, s, c, par, p, q02, e);
+    pair_terms<0, 3, 1, 2>(U, g, s, c, par, p, q03, e);
+    q_x[s] = (-1.0 / (256.0 * M_PI * M_PI)) * (q01 - q02 + q03);
+}
+
 // one thread per timeslice: ordered sum over pairs and chunks
 __global__ void obs_finish_kernel(const double* part, int nchunk, int nt, double* out)
 {
@@ -353,6 +370,16 @@ void k_observables(lt_field u, double* host_out)
     launch(ctx, "observables_finish", obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, (const double*)part,
            (int)nchunk, nt, fin);
     LT_HIP_CHECK(hipMemcpyAsync(host_out, fin, 3 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void k_topcharge_density(lt_field u, double* host_q)
+{
+    lt_ctx ctx = u->ctx;
+    const Geom& g = u->g;
+    double* d = ctx->scratch_for((size_t)g.V);
+    launch(ctx, "topcharge_density", qdensity_kernel, dim3(nblocks(g.V, BS)), dim3(BS), 0, (const double2*)u->d, g, d);
+    LT_HIP_CHECK(hipMemcpyAsync(host_q, d, (size_t)g.V * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 

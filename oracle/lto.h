@@ -72,6 +72,7 @@ double lto_plaquette_sum(const double *U, lto_dims d);
 double lto_wilson_action(const double *U, lto_dims d, double beta);
 void   lto_clover(const double *U, lto_dims d, double *C6);       /* 6 planes per site: 01,02,03,12,13,23 */
 double lto_topcharge_clover(const double *U, lto_dims d);          /* topological_charge.hpp:97-200  */
+void   lto_topcharge_density(const double *U, lto_dims d, double *q_x);  /* per-site summand, :184-199 */
 double lto_topcharge_clover_tensor(const double *C6, lto_dims d);  /* topological_charge.hpp:202-231 */
 void   lto_topcharge_timeslice(const double *U, lto_dims d, double *q_t);   /* :337-364 via F */
 void   lto_energy_clover_timeslice(const double *U, lto_dims d, double *e_t);

@@ -50,6 +50,7 @@ def _load():
         "lto_wilson_action": (ctypes.c_double, [D, dims, ctypes.c_double]),
         "lto_clover": (None, [D, dims, D]),
         "lto_topcharge_clover": (ctypes.c_double, [D, dims]),
+        "lto_topcharge_density": (None, [D, dims, D]),
         "lto_topcharge_clover_tensor": (ctypes.c_double, [D, dims]),
         "lto_topcharge_timeslice": (None, [D, dims, D]),
         "lto_energy_clover_timeslice": (None, [D, dims, D]),
@@ -177,6 +178,13 @@ def clover(U, shape):
 
 def topcharge(U, shape):
     return lib().lto_topcharge_clover(_p(U), dims(shape))
+
+
+def topcharge_density(U, shape):
+    """Per-site summand of TopChargeClover (topological_charge.hpp:184-199), shape (Nt, Nx, Ny, Nz)."""
+    q = np.zeros(tuple(shape))
+    lib().lto_topcharge_density(_p(U), dims(shape), _p(q))
+    return q
 
 
 def topcharge_tensor(C6, shape):

@@ -89,6 +89,19 @@ double lto_topcharge_clover(const double *U, lto_dims d)
     return -1.0 / (64.0 * 4.0 * LTO_PI * LTO_PI) * Q;
 }
 
+/* The per-site summand of TopChargeClover (topological_charge.hpp:184-199): the
+ * terms that lto_topcharge_clover adds up, one per site, with its normalisation. */
+void lto_topcharge_density(const double *U, lto_dims d, double *q_x)
+{
+    size_t V = lto_volume(d);
+    #pragma omp parallel for
+    for (size_t s = 0; s < V; ++s) {
+        mat3 C[6];
+        for (int p = 0; p < 6; ++p) C[p] = clover_leaf_sum(U, d, s, PL_MU[p], PL_NU[p]);
+        q_x[s] = -1.0 / (64.0 * 4.0 * LTO_PI * LTO_PI) * q_density_raw(C);
+    }
+}
+
 /* TopChargeClover(const FullTensor&) (topological_charge.hpp:202-231) */
 double lto_topcharge_clover_tensor(const double *C6, lto_dims d)
 {

@@ -1,0 +1,182 @@
+"""Parity at the workloads' own sizes: 32^4 (the flow / sweep benchmark lattice) and 48^4
+(the large single lattice), through the C ABI against the CPU oracle.
+
+The launch maps of the link kernels depend on the extents (the thread-block tile, the
+chunking of t and y tiles, the spread of blocks over the XCDs), so a link that a
+mis-mapped launch skipped or updated twice would pass any invariant an untouched link
+keeps (OR conserves the action link by link).  These tests compare every link.
+
+Inputs are fields the GPU itself brought near equilibrium (Haar start plus a few heatbath
+and overrelaxation updates), downloaded once; the oracle then starts from the same links.
+Bars: 1e-12 relative for the deterministic passes (overrelaxation, flow, observables),
+1e-11 per link for the heatbath, which shares the Philox draws with the oracle (a
+rejection decision could only flip for a draw within rounding of the acceptance edge).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+L32 = (32, 32, 32, 32)
+L48 = (48, 48, 48, 48)
+
+
+def flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def la():
+    import lettuce_amd
+    return lettuce_amd
+
+
+def thermalised(la, ctx, shape, beta, n_update, seed=0):
+    f = la.GaugeField(ctx, shape)
+    f.SetToHaar(seed)
+    f.set_rng(seed, 0)
+    hb, orx = la.HeatbathKernel(f, beta), la.OverrelaxationSubgroupKernel(f)
+    for _ in range(n_update):
+        la.Iterator.Checkerboard(hb, 1)
+        la.Iterator.Checkerboard(orx, 4)
+    return f
+
+
+@pytest.fixture(scope="module")
+def start32(la, gpu_ctx):
+    flush("32^4 start: Haar(0) + 10 x (HB + 4 OR) at beta 6.0 on the GPU")
+    f = thermalised(la, gpu_ctx, L32, 6.0, 10)
+    U = f.download()
+    f.close()
+    return U
+
+
+def link_diff(a, b):
+    return np.max(np.abs(a - b).reshape(-1, 18), axis=1)
+
+
+@pytest.mark.parametrize("schedule", [2, 4])
+def test_overrelaxation_32(la, lto, gpu_ctx, start32, schedule):
+    flush(f"32^4 OR sweep, schedule {schedule}")
+    f = la.GaugeField(gpu_ctx, L32)
+    f.upload(start32)
+    orx = la.OverrelaxationSubgroupKernel(f)
+    la.Iterator.Checkerboard(orx, 1) if schedule == 2 else la.Iterator.Checkerboard4(orx, 1)
+    got = f.download()
+    f.close()
+    ref = start32.copy()
+    lto.or_sweep(ref, L32, 1, schedule)
+    assert rel_err(got, ref) <= 1e-12
+    # every link moved (an OR update leaves no link of a non-trivial field unchanged)
+    assert np.min(link_diff(ref, start32)) > 0.0 and np.min(link_diff(got, start32)) > 0.0
+
+
+def test_heatbath_32(la, lto, gpu_ctx, start32):
+    flush("32^4 HB sweep (CB2), Philox (5, 3)")
+    f = la.GaugeField(gpu_ctx, L32)
+    f.upload(start32)
+    f.set_rng(5, 3)
+    la.Iterator.Checkerboard(la.HeatbathKernel(f, 6.0), 1)
+    got = f.download()
+    f.close()
+    ref = start32.copy()
+    lto.hb_sweep(ref, L32, 6.0, 1, 2, 5, 3)
+    assert np.max(link_diff(got, ref)) <= 1e-11
+
+
+def test_flow_32(la, lto, gpu_ctx, start32):
+    """10 RK3 steps of eps = 0.01 (the flow measurement's interval), then the field link by
+    link and E_plaq, E_clov, Q, Q_t, E_t."""
+    flush("32^4 RK3 flow, 10 steps")
+    f = la.GaugeField(gpu_ctx, L32)
+    f.upload(start32)
+    g = la.GaugeField(gpu_ctx, L32)
+    fl = la.GradientFlowKernel(f, g, "rk3", 0.01)
+    fl(10)
+    got = g.download()
+    o = la.Observables(g)
+    fl.close()
+    f.close()
+    g.close()
+    ref = lto.flow(start32, L32, 0.01, 10, "rk3")
+    assert rel_err(got, ref) <= 1e-12
+    qt = lto.topcharge_timeslice(ref, L32)
+    et = lto.energy_clover_timeslice(ref, L32)
+    scale = max(1.0, np.sum(np.abs(qt)))
+    assert abs(o["Q"] - qt.sum()) <= 1e-12 * scale
+    assert np.max(np.abs(o["Q_t"] - qt)) <= 1e-12 * scale
+    assert rel_err(o["E_t"], et) <= 1e-12
+    ep = lto.energy_plaquette(ref, L32)
+    assert abs(o["E_plaq"] - ep) <= 1e-12 * abs(ep)
+    assert abs(o["E_clov"] - et.sum()) <= 1e-12 * abs(et.sum())
+
+
+@pytest.fixture(scope="module")
+def start48(la, gpu_ctx):
+    flush("48^4 start: Haar(0) + 3 x (HB + 4 OR) at beta 6.2 on the GPU")
+    f = thermalised(la, gpu_ctx, L48, 6.2, 3)
+    U = f.download()
+    f.close()
+    return U
+
+
+def test_update_48(la, lto, gpu_ctx, start48):
+    """One HB + 4 OR update (CB2) at 48^4, beta 6.2, on the same Philox counters."""
+    flush("48^4 HB + 4 OR update")
+    f = la.GaugeField(gpu_ctx, L48)
+    f.upload(start48)
+    f.set_rng(11, 0)
+    la.Iterator.Checkerboard(la.HeatbathKernel(f, 6.2), 1)
+    la.Iterator.Checkerboard(la.OverrelaxationSubgroupKernel(f), 4)
+    got = f.download()
+    f.close()
+    ref = start48.copy()
+    lto.hb_sweep(ref, L48, 6.2, 1, 2, 11, 0)
+    lto.or_sweep(ref, L48, 4, 2)
+    assert np.max(link_diff(got, ref)) <= 1e-11
+
+
+def test_flow_48(la, lto, gpu_ctx, start48):
+    flush("48^4 RK3 flow, 2 steps")
+    f = la.GaugeField(gpu_ctx, L48)
+    f.upload(start48)
+    g = la.GaugeField(gpu_ctx, L48)
+    fl = la.GradientFlowKernel(f, g, "rk3", 0.01)
+    fl(2)
+    got = g.download()
+    fl.close()
+    f.close()
+    g.close()
+    assert rel_err(got, lto.flow(start48, L48, 0.01, 2, "rk3")) <= 1e-12
+
+
+def test_checkpointed_cv_48(la, gpu_ctx, start48):
+    """The metadynamics CV at 48^4 with 8 stout levels: per-level checkpointing (every 3rd
+    level kept, the rest recomputed) gives the charge and the force of full storage bit for
+    bit."""
+    flush("48^4 CV, 8 levels: full storage")
+    U = la.GaugeField(gpu_ctx, L48)
+    U.upload(start48)
+    F = la.MatrixField(gpu_ctx, L48)
+    cv = la.CollectiveVariable(gpu_ctx, L48, 8, 0.12)
+    full_bytes = cv.bytes()
+    q_full = cv.forward(U)
+    cv.backward(0.75, F)
+    cv.close()
+    F_full = F.download()
+    flush("48^4 CV, 8 levels: checkpoint interval 3")
+    cv = la.CollectiveVariable(gpu_ctx, L48, 8, 0.12, checkpoint=3)
+    ckpt_bytes = cv.bytes()
+    q_ckpt = cv.forward(U)
+    cv.backward(0.75, F)
+    cv.close()
+    F_ckpt = F.download()
+    U.close()
+    F.close()
+    flush(f"  workspace {full_bytes / 1e9:.1f} GB full, {ckpt_bytes / 1e9:.1f} GB checkpointed")
+    assert ckpt_bytes < full_bytes
+    assert q_full == q_ckpt
+    assert np.array_equal(F_full, F_ckpt)
+    assert np.max(np.abs(F_full)) > 0.0

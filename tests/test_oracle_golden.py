@@ -160,3 +160,15 @@ def gell_mann():
     l[7][0, 0] = l[7][1, 1] = 1 / np.sqrt(3)
     l[7][2, 2] = -2 / np.sqrt(3)
     return l
+
+
+@pytest.mark.parametrize("name", ["haar", "smooth"])
+def test_charge_density_sums_to_reference_timeslices(lto, golden, start_fields, name):
+    """The per-site charge density adds up, timeslice by timeslice, to the reference's
+    TopChargeCloverTimeslice and in total to its TopChargeClover (16^4)."""
+    q = lto.topcharge_density(start_fields[name], SHAPE)
+    ref = golden[f"obs_{name}"]
+    nt = SHAPE[0]
+    scale = max(1.0, np.sum(np.abs(ref[7:7 + nt])))
+    assert np.max(np.abs(q.sum(axis=(1, 2, 3)) - ref[7:7 + nt])) <= 1e-12 * scale
+    assert abs(q.sum() - ref[2]) <= 1e-12 * scale

@@ -247,3 +247,18 @@ def test_heatbath_cap_returns_identity(lto):
     assert att == [1, 1, 1]
     M = links(U)[5 * 4 + 2]
     assert np.max(np.abs(links(W)[5 * 4 + 2] - M)) < 1e-14
+
+
+def test_charge_density_is_local(lto):
+    """The per-site density is gauge invariant site by site and moves with the field:
+    shifting the lattice by one site along mu shifts the density the same way (pins the
+    site order that timeslice sums leave open)."""
+    shape = (4, 6, 4, 6)
+    U = lto.stout(lto.haar(shape, 3), shape, 0.1, 2)
+    q = lto.topcharge_density(U, shape)
+    W = gauge_transform(U, random_gauge(shape, 4, lto), shape)
+    assert np.max(np.abs(lto.topcharge_density(W, shape) - q)) < 1e-15
+    M = U.reshape(shape + (4, 18))
+    for mu in range(4):
+        S = np.ascontiguousarray(np.roll(M, 1, axis=mu)).reshape(U.shape)
+        assert np.max(np.abs(lto.topcharge_density(S, shape) - np.roll(q, 1, axis=mu))) < 1e-16
