@@ -67,8 +67,8 @@ def cpu_baseline(L, beta, budget_s):
         t_tot += time.perf_counter() - t0
         n_upd += 1
     links = 5 * 4 * L ** 4 * n_upd
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": links / t_tot, "unit": "link updates/s", "cores": threads, "kind": "port",
+    return {"value": links / t_tot, "unit": "link updates/s", "cores": lto.threads(), "kind": "port",
+            "host_cpus": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0)),
             "sample": f"{n_upd} x (1 HB + 4 OR) sweeps of {L}^4 by the OpenMP C oracle, {t_tot:.1f} s"}
 
 

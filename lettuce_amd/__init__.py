@@ -737,6 +737,13 @@ class MetadynamicsTemperingKernel:
         self.acceptances += acc.value
         return bool(acc.value)
 
+    def get_rng(self):
+        """(seed, counter) of the swap proposals' Philox stream (saved by PhiloxState)."""
+        return self.seed, self.counter
+
+    def set_rng(self, seed, counter=0):
+        self.seed, self.counter = int(seed), int(counter)
+
 
 class _ReplicaField(GaugeField):
     """Gauge field of replica r of a ReplicaBatch; the batch owns its memory and

@@ -62,31 +62,44 @@ def LoadConfigBMWFull(U, filename):
 
 
 class PhiloxState:
-    """Generator state of a field (or of one replica of a batch): seed and counter of
-    its Philox stream, the counterpart of PRNG4D::SaveState / LoadState."""
+    """Generator state of one or more Philox streams: fields (or replicas of a batch),
+    whose seed and counter drive their heatbath draws, momenta and Metropolis draws, and
+    MetadynamicsTemperingKernel swap proposals -- the counterpart of PRNG4D::SaveState /
+    LoadState, which keeps all of a run's draws in one saved generator.  The file holds
+    one line "philox4x32-10 <seed> <counter>" per holder, in the order given."""
 
     TAG = "philox4x32-10"
 
-    def __init__(self, field):
-        self.field = field
+    def __init__(self, *holders):
+        if not holders:
+            raise ValueError("PhiloxState needs at least one field or tempering kernel")
+        self.holders = holders
+        self.field = holders[0]
 
     def SaveState(self, filename_prng, filename_distribution, overwrite=False):
         for f in (filename_prng, filename_distribution):
             if os.path.exists(f) and not overwrite:
                 return False
-        seed, counter = self.field.get_rng()
+        lines = []
+        for h in self.holders:
+            seed, counter = h.get_rng()
+            lines.append(f"{self.TAG} {seed} {counter}\n")
         with open(filename_prng, "w") as f:
-            f.write(f"{self.TAG} {seed} {counter}\n")
+            f.writelines(lines)
         with open(filename_distribution, "w") as f:
             f.write(f"{self.TAG} stateless\n")
         return True
 
     def LoadState(self, filename_prng, filename_distribution=None):
         with open(filename_prng) as f:
-            tag, seed, counter = f.read().split()
-        if tag != self.TAG:
-            raise ValueError(f"{filename_prng}: not a {self.TAG} state")
-        self.field.set_rng(int(seed), int(counter))
+            rows = [line.split() for line in f if line.strip()]
+        if len(rows) != len(self.holders):
+            raise ValueError(f"{filename_prng}: {len(rows)} generator states for {len(self.holders)} holders")
+        for row in rows:
+            if len(row) != 3 or row[0] != self.TAG:
+                raise ValueError(f"{filename_prng}: not a {self.TAG} state")
+        for h, (_, seed, counter) in zip(self.holders, rows):
+            h.set_rng(int(seed), int(counter))
         return True
 
 

@@ -38,6 +38,7 @@ double lto_u01(uint32_t a, uint32_t b);
 
 /* ---- lattice ------------------------------------------------------------- */
 size_t lto_volume(lto_dims d);
+int    lto_threads(void);                                   /* OpenMP threads per region */
 void   lto_set_identity(double *U, lto_dims d);
 void   lto_haar(double *U, lto_dims d, uint64_t seed);
 void   lto_reunitarize(double *U, lto_dims d);

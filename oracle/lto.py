@@ -35,6 +35,7 @@ def _load():
         "lto_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
         "lto_u01": (ctypes.c_double, [ctypes.c_uint32, ctypes.c_uint32]),
         "lto_volume": (ctypes.c_size_t, [dims]),
+        "lto_threads": (ctypes.c_int, []),
         "lto_set_identity": (None, [D, dims]),
         "lto_haar": (None, [D, dims, ctypes.c_uint64]),
         "lto_reunitarize": (None, [D, dims]),
@@ -103,6 +104,11 @@ def _p(a):
 def dims(shape):
     nt, nx, ny, nz = shape
     return Dims(nt, nx, ny, nz)
+
+
+def threads():
+    """OpenMP threads of the oracle's parallel regions."""
+    return lib().lto_threads()
 
 
 def volume(shape):

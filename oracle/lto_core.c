@@ -62,6 +62,9 @@ static inline void lto_gauss2(const double u[2], double g[2])
 /* ------------------------------------------------------------------------- */
 size_t lto_volume(lto_dims d) { return (size_t)d.nt * d.nx * d.ny * d.nz; }
 
+/* OpenMP threads a parallel region of the oracle uses */
+int lto_threads(void) { return omp_get_max_threads(); }
+
 /* GaugeField4D::SetToIdentity (lattice.hpp:170-183) */
 void lto_set_identity(double *U, lto_dims d)
 {

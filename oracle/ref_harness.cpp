@@ -432,6 +432,33 @@ int main(int argc, char** argv)
             series.push_back(PlaquetteSum(*U) / (6.0 * 3.0 * U->Volume()));
         }
         write_values(argv[2], series);
+    } else if (cmd == "hb_qstats") {             // hb_qstats out beta n_update n_or seed n_stout rho
+        // As hb_stats, recording per update the average plaquette and the clover charge
+        // after n_stout StoutSmearing4D levels (the metadynamics CV, hmc_metadynamics.hpp:
+        // 302-310), from a cold start: pairs (P, Q) per update.
+        set_beta(std::atof(argv[3]));
+        int nupd = std::atoi(argv[4]);
+        int nor = std::atoi(argv[5]);
+        pcg64 g(std::strtoull(argv[6], nullptr, 10));
+        int nst = std::atoi(argv[7]);
+        double rho = std::atof(argv[8]);
+        PRNG4D<Nt, Nx, Ny, Nz, pcg64, floatT, int> prng(g);
+        U->SetToIdentity();
+        auto act = GaugeAction::WilsonAction;
+        HeatbathKernel HB(*U, act, prng);
+        OverrelaxationSubgroupKernel ORk(*U, act);
+        auto A = std::make_unique<GaugeField>();
+        auto B = std::make_unique<GaugeField>();
+        std::vector<double> series;
+        for (int k = 0; k < nupd; ++k) {
+            Iterator::Checkerboard4(HB, 1);
+            Iterator::Checkerboard4(ORk, nor);
+            series.push_back(PlaquetteSum(*U) / (6.0 * 3.0 * U->Volume()));
+            *A = *U;
+            for (int l = 0; l < nst; ++l) { StoutSmearing4D(*A, *B, rho); std::swap(A, B); }
+            series.push_back(TopChargeClover(*A));
+        }
+        write_values(argv[2], series);
     } else if (cmd == "time") {                  // time in beta nrep: seconds per HB sweep, OR sweep (CB2), RK3 step
         read_field(argv[2], *U);
         set_beta(std::atof(argv[3]));

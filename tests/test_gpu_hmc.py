@@ -491,6 +491,48 @@ def test_tempering_run_matches_oracle(la, lto, gpu_ctx):
     cv.close()
 
 
+def test_tempering_run_resumes_bit_for_bit(la, lto, gpu_ctx, tmp_path):
+    """A two-stream TemperingRun stopped after two iterations and resumed from saved
+    state (both fields, the bias grid and current CV, and one PhiloxState holding the
+    unbiased field's, the MetaD field's and the swap proposals' streams) continues
+    exactly as the uninterrupted run."""
+    flush("tempering run resume")
+    shape = (4, 4, 4, 8)
+    rho, nl, beta, tau, n_hmc = 0.12, 2, 6.0, 0.3, 3
+
+    def make(A, B):
+        U, Ut = gauge(la, gpu_ctx, shape, A), gauge(la, gpu_ctx, shape, B)
+        Uc, P, F = gauge(la, gpu_ctx, shape), matrix(la, gpu_ctx, shape), matrix(la, gpu_ctx, shape)
+        bias = la.MetaBiasPotential(-8.0, 8.0, 800, 1.5, 0.3, 10.0, 1000.0, well_tempered=True)
+        cv = la.CollectiveVariable(gpu_ctx, shape, nl, rho)
+        hmc = la.HMCMetaDKernel(Ut, Uc, P, F, bias, cv, "omf2", beta, tau, update_bias=True)
+        temper = la.MetadynamicsTemperingKernel(U, Ut, bias, cv, seed=77)
+        run = la.TemperingRun(la.HeatbathKernel(U, beta), la.OverrelaxationSubgroupKernel(U), hmc, temper,
+                              n_heatbath=1, n_orelax=2, tempering_nonmetadynamics_sweeps=1, tempering_swap_period=1)
+        return run, U, Ut, bias, temper, cv
+
+    A, B = smooth(lto, shape, 41, 2), smooth(lto, shape, 42, 2)
+    run, U, Ut, bias, temper, cv = make(A, B)
+    U.set_rng(5, 0)
+    Ut.set_rng(6, 0)
+    for n in range(2):
+        run.step(n, n_hmc, True)
+    saved = {"U": U.download(), "Ut": Ut.download(), "grid": bias.grid(), "cv": bias.ReturnCV_current()}
+    assert la.PhiloxState(U, Ut, temper).SaveState(str(tmp_path / "prng.txt"), str(tmp_path / "dist.txt"))
+    trail = [run.step(n, n_hmc, True) for n in range(2, 5)]
+    end = (U.download(), Ut.download(), bias.grid())
+    cv.close()
+    run2, U2, Ut2, bias2, temper2, cv2 = make(saved["U"], saved["Ut"])
+    bias2.set_grid(saved["grid"])
+    bias2.SetCV_current(saved["cv"])
+    assert la.PhiloxState(U2, Ut2, temper2).LoadState(str(tmp_path / "prng.txt"))
+    trail2 = [run2.step(n, n_hmc, True) for n in range(2, 5)]
+    assert trail2 == trail
+    assert np.array_equal(U2.download(), end[0]) and np.array_equal(Ut2.download(), end[1])
+    assert np.array_equal(bias2.grid(), end[2])
+    cv2.close()
+
+
 def test_metad_run_parity_coin(la, lto, golden, gpu_ctx):
     """MetaDRun (SU3_main.cpp:523-531): a trajectory, then ParityUpdate on heads."""
     flush("metad run parity")

@@ -217,8 +217,9 @@ def test_flow(la, lto, gpu_ctx, integrator):
 
 @pytest.mark.parametrize("shape", [(4, 6, 8, 16), (2, 4, 6, 32), (4, 4, 4, 12)])
 def test_flow_shapes(la, lto, gpu_ctx, shape):
-    """RK3 flow on extents that take the LDS-plane stage (Nz = 16, 32; 4 and 2 rows per
-    block, odd Nx) and the tiled stage (Nz = 12)."""
+    """RK3 flow on uneven extents: long z rows (Nz = 16, 32) with short t, x, y and a
+    z extent (12) that a power-of-two tile does not divide, so the thread-block tile,
+    the t / y chunking and the XCD remap all take shapes other than the cubic ones."""
     flush(f"flow rk3 {shape}")
     U = smooth(lto, shape, 3)
     f = field(la, gpu_ctx, shape, U)

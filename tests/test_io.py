@@ -172,3 +172,31 @@ def test_checkpoint_rotation(tmp_path):
     assert (tmp_path / "config_temper.conf").read_text() == "gen 4"
     cm.CreateCompletedFile(tmp_path)
     assert (tmp_path / "completed_run").exists()
+
+
+def test_philox_state_of_several_streams(tmp_path):
+    """A two-stream tempering run keeps three Philox streams (the unbiased field's, the
+    MetaD field's and the swap proposals'); one state file holds all of them."""
+    import lettuce_amd as la
+
+    class Holder:
+        def __init__(self, seed, counter):
+            self.rng = (seed, counter)
+
+        def get_rng(self):
+            return self.rng
+
+        def set_rng(self, seed, counter=0):
+            self.rng = (seed, counter)
+
+    a, b, c = Holder(5, 12), Holder(6, 3), Holder(77, 2)
+    st = la.PhiloxState(a, b, c)
+    assert st.SaveState(str(tmp_path / "p.txt"), str(tmp_path / "d.txt"))
+    assert not st.SaveState(str(tmp_path / "p.txt"), str(tmp_path / "d.txt"))
+    assert (tmp_path / "p.txt").read_text().splitlines() == ["philox4x32-10 5 12", "philox4x32-10 6 3",
+                                                             "philox4x32-10 77 2"]
+    a.rng = b.rng = c.rng = (0, 0)
+    assert st.LoadState(str(tmp_path / "p.txt"))
+    assert (a.rng, b.rng, c.rng) == ((5, 12), (6, 3), (77, 2))
+    with pytest.raises(ValueError):
+        la.PhiloxState(a, b).LoadState(str(tmp_path / "p.txt"))
