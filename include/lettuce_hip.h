@@ -169,6 +169,17 @@ int lt_energy(lt_field u, double *e_plaq, double *e_clov, double *e_t);
 /* all of the above in one fused pass: out[0]=plaquette sum, out[1]=Q, out[2]=E_plaq,
  * out[3]=E_clov, then q_t[Nt], e_t[Nt]; out must hold 4 + 2*Nt doubles */
 int lt_observables(lt_field u, double *out);
+/* lt_observables without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device or pinned memory (tight measurement loops,
+ * harnesses that forbid hipMalloc).  The partial sums live in a fixed device array of
+ * 8 MiB per GPU (held until the process ends); the final values are computed on the
+ * device and copied into out, which must hold 4 + 2*Nt doubles; the call synchronises
+ * the context stream (it is not for graph capture).  Results equal lt_observables' bit
+ * for bit.  Lattices whose partial sums exceed the
+ * array (9*ceil(Nx*Ny*Nz/256)*Nt + 5*Nt + 4 doubles > 2^20: 72^4 fits, 76^4 does not)
+ * return LT_EINVAL.  Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other threads, other contexts' streams) race. */
+int lt_observables_noalloc(lt_field u, double *out);
 
 /* ---- smearing -------------------------------------------------------------- */
 /* StoutSmearing4D applied n times (stout_smearing.hpp:29-57, StoutSmearingN :61-75);

@@ -48,6 +48,7 @@ SIGNATURES = {
     "lt_topcharge_clover_density": [c_field, D],
     "lt_energy": [c_field, D, D, D],
     "lt_observables": [c_field, D],
+    "lt_observables_noalloc": [c_field, D],
     "lt_stout": [c_field, c_field, ctypes.c_double, I],
     "lt_flow_create": [c_field, c_field, I, ctypes.c_double, ctypes.POINTER(c_flow)],
     "lt_flow_destroy": [c_flow],

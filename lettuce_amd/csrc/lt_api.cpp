@@ -198,6 +198,7 @@ int lt_ctx_create(int device, lt_ctx* out)
         LT_HIP_CHECK(hipGetDeviceCount(&n));
         if (device < 0 || device >= n) return fail(LT_EINVAL, "lt_ctx_create: no such device");
         LT_HIP_CHECK(hipSetDevice(device));
+        lt::noalloc_setup(device);
         auto* c = new lt_ctx_s();
         c->device = device;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -599,6 +600,17 @@ int lt_observables(lt_field u, double* out)
         out[1] = q;
         out[2] = 2.0 * (18.0 - p / V);
         out[3] = e;
+        return LT_OK;
+    });
+}
+
+int lt_observables_noalloc(lt_field u, double* out)
+{
+    if (!u || !out) return fail(LT_EINVAL, "null argument");
+    if (!u->su3) return fail(LT_ESTATE, "lt_observables_noalloc: the field must hold SU(3) links");
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_observables_noalloc(u, out);
         return LT_OK;
     });
 }

@@ -243,6 +243,11 @@ void k_heatbath_sweep_action(lt_field u, double beta, int n, double c_plaq, doub
 void k_observables(lt_field u, double* host_out);
 // per-site clover charge density (V doubles, lexicographic order) into host memory
 void k_topcharge_density(lt_field u, double* host_q);
+// the fixed observables scratch of lt_observables_noalloc: its address on `device`
+// (which must be current), looked up once per device
+void noalloc_setup(int device);
+// lt_observables' sums through the fixed scratch, final values computed on the device
+void k_observables_noalloc(lt_field u, double* host_out);
 void k_rect_sums(lt_field u, double out[3]);
 double k_wilson_loop(lt_field u, lt_field chain, int n0, int n1, int reset);
 void k_polyakov_loop(lt_field u, double* re, double* im);

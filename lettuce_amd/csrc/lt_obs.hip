@@ -5,6 +5,8 @@
 // chunk, then an ordered sum over chunks.
 #include "lt_internal.hpp"
 
+#include <mutex>
+
 namespace lt {
 
 namespace {
@@ -370,6 +372,100 @@ void k_observables(lt_field u, double* host_out)
     launch(ctx, "observables_finish", obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, (const double*)part,
            (int)nchunk, nt, fin);
     LT_HIP_CHECK(hipMemcpyAsync(host_out, fin, 3 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+// ---- observables without allocation ---------------------------------------------
+// lt_observables_noalloc keeps its partial sums in this fixed device array instead of
+// the context's scratch.  Host code uses only the per-device address that
+// hipGetSymbolAddress returns (the array's name is a host-side stand-in).
+namespace {
+
+constexpr size_t NOALLOC_DOUBLES = (size_t)1 << 20;   // 8 MiB per GPU
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+std::mutex noalloc_mutex;
+double* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The per-timeslice sums -> [plaq_sum, Q, E_plaq, E_clov, q_t[Nt], e_t[Nt]] in one
+// thread, operation for operation as lt_observables does it on the host (no
+// contraction), so both entry points return the same bits.
+__global__ void obs_final_kernel(const double* __restrict__ h, int nt, double V, double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double qnorm = -1.0 / (256.0 * M_PI * M_PI);
+    double p = 0.0, q = 0.0, e = 0.0;
+    for (int t = 0; t < nt; ++t) {
+        const double qt = qnorm * h[3 * t + 1];
+        const double et = h[3 * t + 2] / V;
+        out[4 + t] = qt;
+        out[4 + nt + t] = et;
+        p += h[3 * t];
+        q += qt;
+        e += et;
+    }
+    out[0] = p;
+    out[1] = q;
+    out[2] = 2.0 * (18.0 - p / V);
+    out[3] = e;
+}
+
+}  // namespace
+
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        throw LtError{LT_EINVAL, "device index beyond the fixed observables scratch table"};
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (noalloc_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw LtError{LT_EHIP, std::string("hipGetSymbolAddress(observables scratch): ") + hipGetErrorString(e)};
+    noalloc_ptr[device] = (double*)p;
+}
+
+size_t noalloc_observables_doubles(const Geom& g)
+{
+    const int nt = g.L[0];
+    const size_t nchunk = nblocks(g.V / nt, BS);
+    return 9 * nchunk * nt + 3 * (size_t)nt + 4 + 2 * (size_t)nt;
+}
+
+void k_observables_noalloc(lt_field u, double* host_out)
+{
+    lt_ctx ctx = u->ctx;
+    const Geom& g = u->g;
+    const size_t need = noalloc_observables_doubles(g);
+    if (need > NOALLOC_DOUBLES)
+        throw LtError{LT_EINVAL, "lt_observables_noalloc: the lattice needs " + std::to_string(need) +
+                                     " doubles of scratch, the fixed array holds " + std::to_string(NOALLOC_DOUBLES)};
+    double* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(noalloc_mutex);
+        if (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) base = noalloc_ptr[ctx->device];
+    }
+    if (!base) throw LtError{LT_ESTATE, "lt_observables_noalloc: no scratch address for this device"};
+    const int nt = g.L[0];
+    const int Vs = g.V / nt;
+    const unsigned nchunk = nblocks(Vs, BS);
+    double* part = base;
+    double* fin = part + 9 * (size_t)nchunk * nt;
+    double* out = fin + 3 * (size_t)nt;
+    // every partial and every per-timeslice sum is written before it is read: nothing
+    // has to start at zero
+    hipLaunchKernelGGL(obs_kernel, dim3(3 * nchunk * nt), dim3(BS), 0, ctx->stream, (const double2*)u->d, g, Vs,
+                       (int)nchunk, part);
+    LT_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, ctx->stream, (const double*)part,
+                       (int)nchunk, nt, fin);
+    LT_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(obs_final_kernel, dim3(1), dim3(64), 0, ctx->stream, (const double*)fin, nt, (double)g.V, out);
+    LT_HIP_CHECK(hipGetLastError());
+    LT_HIP_CHECK(hipMemcpyAsync(host_out, out, (4 + 2 * (size_t)nt) * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
     LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
