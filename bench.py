@@ -179,24 +179,42 @@ def pt_metad(la, ctx, shapes, n_step=20, steps=2, graph=True):
 
 def lattice8(la, ctx, n_update=200):
     """8^4 at beta = 5.7: n_update HB + 4 OR updates from Haar(0), plaquette after each (the
-    correctness configuration: launch-latency bound)."""
+    correctness configuration: launch-latency bound).  Timed twice from the same start:
+    one call per sweep with a plaquette read-back per update, and lt_update_sweeps (the
+    update captured once as a graph, plaquettes read back after the last update); both
+    give the same field bit for bit."""
     shape = (8,) * 4
-    U = la.GaugeField(ctx, shape)
-    U.SetToHaar(0)
-    U.set_rng(0, 0)
-    hb = la.HeatbathKernel(U, 5.7)
-    orx = la.OverrelaxationSubgroupKernel(U)
-    plaq = []
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(n_update):
-        la.Iterator.Checkerboard(hb, 1)
-        la.Iterator.Checkerboard(orx, 4)
-        plaq.append(la.PlaquetteSum(U) / (18.0 * 8 ** 4))
-    dt = time.perf_counter() - t0
-    U.close()
-    return {"lattice": list(shape), "beta": 5.7, "updates": n_update, "ms_per_update": 1000.0 * dt / n_update,
-            "mean_plaquette_last_100": float(np.mean(plaq[-100:]))}
+    out = {"lattice": list(shape), "beta": 5.7, "updates": n_update}
+    fields = []
+    for mode in ("eager", "graph"):
+        U = la.GaugeField(ctx, shape)
+        U.SetToHaar(0)
+        U.set_rng(0, 0)
+        if mode == "graph":
+            la.UpdateSweeps(U, 5.7, 1, 4, 1, la.CB2)     # capture outside the timed loop
+            U.SetToHaar(0)
+            U.set_rng(0, 0)
+        hb = la.HeatbathKernel(U, 5.7)
+        orx = la.OverrelaxationSubgroupKernel(U)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        if mode == "eager":
+            plaq = []
+            for _ in range(n_update):
+                la.Iterator.Checkerboard(hb, 1)
+                la.Iterator.Checkerboard(orx, 4)
+                plaq.append(la.PlaquetteSum(U))
+            plaq = np.array(plaq)
+        else:
+            plaq = la.UpdateSweeps(U, 5.7, 1, 4, n_update, la.CB2)
+        dt = time.perf_counter() - t0
+        out[f"ms_per_update_{mode}"] = 1000.0 * dt / n_update
+        out[f"mean_plaquette_last_100_{mode}"] = float(np.mean(plaq[-100:]) / (18.0 * 8 ** 4))
+        fields.append(U.download())
+        U.close()
+    out["ms_per_update"] = out["ms_per_update_graph"]
+    out["graph_equals_eager"] = bool(np.array_equal(fields[0], fields[1]))
+    return out
 
 
 def lattice48(la, ctx, n_update=200, flow_steps=800, interval=3):
@@ -457,6 +475,8 @@ def main():
         "thermalisation_s": t_therm,
     }
     U.close()
+    print("# 8^4 correctness configuration", file=sys.stderr, flush=True)
+    result["lattice_8"] = lattice8(la, ctx)
     if not args.no_pt:
         print("# PT-MetaD 16 x 16^4", file=sys.stderr, flush=True)
         g16 = (16, 16, 16, 16)
@@ -467,8 +487,6 @@ def main():
         mixed = [g16] * 16 + [(12,) * 4, (20,) * 4, (24,) * 4]
         result["pt_metad_mixed"] = pt_metad(la, ctx, mixed, graph=not args.eager)
         result["pt_metad_8x4"] = pt_metad(la, ctx, [(8,) * 4] * 16, graph=not args.eager)
-        print("# 8^4 correctness configuration", file=sys.stderr, flush=True)
-        result["lattice_8"] = lattice8(la, ctx)
     if not args.no_48:
         print("# 48^4", file=sys.stderr, flush=True)
         result["lattice_48"] = lattice48(la, ctx)

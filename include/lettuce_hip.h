@@ -135,6 +135,16 @@ int lt_parity_update(lt_field u, lt_field u_copy);
 /* n sweeps of HeatbathKernel (heatbath.hpp:30-130) at coupling beta; draws come
  * from the field's Philox state, whose counter advances by one per sweep */
 int lt_heatbath_sweep(lt_field u, double beta, int n_sweep, int schedule);
+/* n_update repetitions of the local-update loop body of the reference's main program
+ * (SU3_main.cpp:358-385: Iterator::Checkerboard[4](Heatbath, n_heatbath), then
+ * Iterator::Checkerboard[4](OverrelaxationSubgroup, n_orelax)), with PlaquetteSum after
+ * every update written to plaq_sum[n_update] (may be NULL).  One update is captured
+ * once as a hipGraph (the Philox counter and the output slot live in device memory)
+ * and replayed per update, for small lattices whose sweeps are launch-bound; the field
+ * equals that of the same lt_heatbath_sweep / lt_or_sweep calls bit for bit, and the
+ * field's counter advances by n_hb per update.  The plaquette sums come from a
+ * per-site reduction (they agree with lt_plaquette_sum to rounding). */
+int lt_update_sweeps(lt_field u, double beta, int n_hb, int n_or, int schedule, int n_update, double *plaq_sum);
 
 /* ---- observables ---------------------------------------------------------- */
 /* PlaquetteSum (plaquette.hpp:26-43) */

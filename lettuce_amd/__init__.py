@@ -37,6 +37,7 @@ __all__ = [
     "ParityUpdate", "MetadynamicsTemperingKernel", "LuescherWeiszAction", "IwasakiAction", "DBW2Action",
     "StapleAction", "IntegratorSchedule", "HMCKernel", "HMCMetaDKernel", "MetaBiasPotential",
     "CollectiveVariable", "MatrixField", "ReplicaBatch", "CloverDerivative", "MetaDRun", "TemperingRun",
+    "UpdateSweeps",
 ]
 
 CB2 = 2
@@ -261,6 +262,16 @@ class Iterator:
     @staticmethod
     def Checkerboard4(kernel, n_sweep=1):
         kernel.sweep(n_sweep, CB4)
+
+
+def UpdateSweeps(U, beta, n_heatbath=1, n_orelax=4, n_update=1, schedule=CB4, measure=True):
+    """n_update iterations of the local-update loop body of SU3_main.cpp:358-385
+    (Checkerboard[4] heatbath then overrelaxation sweeps), replayed as one captured graph
+    per update; returns PlaquetteSum after each update (None when measure is False)."""
+    out = np.zeros(n_update) if measure else None
+    call("lt_update_sweeps", U.handle, beta, n_heatbath, n_orelax, schedule, n_update,
+         _dp(out) if measure else None)
+    return out
 
 
 def Observables(U):

@@ -42,6 +42,7 @@ SIGNATURES = {
     "lt_staple_field": [c_field, c_field],
     "lt_or_sweep": [c_field, I, I],
     "lt_heatbath_sweep": [c_field, ctypes.c_double, I, I],
+    "lt_update_sweeps": [c_field, ctypes.c_double, I, I, I, I, D],
     "lt_plaquette_sum": [c_field, D],
     "lt_wilson_action": [c_field, ctypes.c_double, D],
     "lt_topcharge_clover": [c_field, D, D],

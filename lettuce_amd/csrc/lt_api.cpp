@@ -316,6 +316,7 @@ int lt_field_destroy(lt_field f)
     return guarded([&] {
         use_device(f->ctx);
         LT_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+        lt::drop_update_graph(f);
         if (f->d) LT_HIP_CHECK(hipFree(f->d));
         f->ctx->live -= 1;
         delete f;
@@ -518,6 +519,20 @@ int lt_heatbath_sweep(lt_field u, double beta, int n_sweep, int schedule)
     return guarded([&] {
         use_device(u->ctx);
         lt::k_hb_sweep(u, beta, n_sweep, schedule);
+        return LT_OK;
+    });
+}
+
+int lt_update_sweeps(lt_field u, double beta, int n_hb, int n_or, int schedule, int n_update, double* plaq_sum)
+{
+    if (!u || n_hb < 0 || n_or < 0 || n_update < 0 || !(beta > 0.0))
+        return fail(LT_EINVAL, "lt_update_sweeps: bad argument");
+    int rc = check_schedule(u, schedule);
+    if (rc) return rc;
+    if (n_update == 0) return LT_OK;
+    return guarded([&] {
+        use_device(u->ctx);
+        lt::k_update_graph(u, beta, n_hb, n_or, schedule, n_update, plaq_sum);
         return LT_OK;
     });
 }

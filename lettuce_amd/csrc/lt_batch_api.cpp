@@ -96,7 +96,10 @@ struct lt_batch_s {
     void release()
     {
         drop_graph();
-        for (auto v : views) delete v;
+        for (auto v : views) {
+            lt::drop_update_graph(v);
+            delete v;
+        }
         views.clear();
         for (size_t k = 1; k < levels.size(); ++k) (void)hipFree(levels[k]);
         for (auto p : tapes) (void)hipFree(p);

@@ -80,6 +80,10 @@ struct lt_ctx_s {
     void collect_profile();
 };
 
+namespace lt {
+struct UpdateGraph;   // captured local-update loop body of a field (lt_update_sweeps)
+}
+
 struct lt_field_s {
     lt_ctx ctx = nullptr;
     lt::Geom g{};
@@ -89,6 +93,7 @@ struct lt_field_s {
     double2* d = nullptr;      // 4 * 2 * 9 * Vh complex doubles
     uint64_t seed = 0;
     uint64_t counter = 0;
+    lt::UpdateGraph* ug = nullptr;
     size_t bytes() const { return (size_t)72 * g.Vh * sizeof(double2); }
 };
 
@@ -230,6 +235,10 @@ void k_check(lt_field f, double* max_dev, long long* nonfinite);
 void k_staple_field(lt_field u, lt_field out);
 void k_or_sweep(lt_field u, int n, int schedule);
 void k_hb_sweep(lt_field u, double beta, int n, int schedule);
+// n_update x (n_hb heatbath + n_or overrelaxation sweeps) as one graph replayed per
+// update, PlaquetteSum after each into plaq_host (may be null); and its teardown
+void k_update_graph(lt_field u, double beta, int n_hb, int n_or, int schedule, int n_update, double* plaq_host);
+void drop_update_graph(lt_field f);
 // fused observables: per-timeslice plaquette sum, raw q, E_clov numerator -> host[3*Nt]
 // BMW sites [n0, n0 + count) <-> big-endian file words (4 * K per site, K = 12 or
 // 18), with per-block checksum partials (A, B) and, on unpacking, row-2 deviations

@@ -290,3 +290,41 @@ def test_large_lattice_properties(la, gpu_ctx):
     o2 = la.Observables(g)
     assert o1["plaquette_sum"] == o2["plaquette_sum"] and o1["Q"] == o2["Q"]
     assert 0.45 < o1["plaquette_sum"] / (18 * f.volume) < 0.7
+
+
+@pytest.mark.parametrize("schedule,shape", [(4, SMALL), (2, SMALL), (2, MIXED)])
+def test_update_sweeps_graph_equals_eager(la, gpu_ctx, schedule, shape):
+    """lt_update_sweeps (the update loop body captured once and replayed per update) against
+    the same heatbath / overrelaxation calls made one by one: the same field bit for bit,
+    the same Philox counter, PlaquetteSum after every update; a second call with more
+    updates (larger output), and one after the field's links were swapped with another
+    field's (the graph is captured again for the new buffer)."""
+    flush(f"update graph, schedule {schedule}, {shape}")
+    a, b = la.GaugeField(gpu_ctx, shape), la.GaugeField(gpu_ctx, shape)
+    for f in (a, b):
+        f.SetToHaar(21)
+        f.set_rng(9, 4)
+    hb, orx = la.HeatbathKernel(a, 5.7), la.OverrelaxationSubgroupKernel(a)
+    sweep = la.Iterator.Checkerboard4 if schedule == 4 else la.Iterator.Checkerboard
+    for n_update in (3, 7):
+        eager = []
+        for _ in range(n_update):
+            sweep(hb, 1)
+            sweep(orx, 4)
+            eager.append(la.PlaquetteSum(a))
+        got = la.UpdateSweeps(b, 5.7, 1, 4, n_update, schedule)
+        assert np.array_equal(a.download(), b.download())
+        assert a.get_rng() == b.get_rng()
+        assert np.max(np.abs(got - np.array(eager))) <= 1e-13 * 18 * a.volume
+    c = la.GaugeField(gpu_ctx, shape)
+    c.SetToIdentity()
+    la.Swap(b, c)     # b now holds the identity, c the evolved links (Swap moves links only)
+    c.set_rng(*a.get_rng())
+    la.UpdateSweeps(c, 5.7, 1, 2, 2, schedule, measure=False)
+    la.UpdateSweeps(b, 5.7, 1, 2, 1, schedule, measure=False)   # b's graph was captured for another buffer
+    for _ in range(2):
+        sweep(hb, 1)
+        sweep(orx, 2)
+    assert np.array_equal(a.download(), c.download())
+    for f in (a, b, c):
+        f.close()
