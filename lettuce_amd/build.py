@@ -17,7 +17,8 @@ LIB = os.path.join(HERE, "liblettuce_hip.so")
 ARCH = os.environ.get("LT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["lt_api.cpp", "lt_hmc_api.cpp", "lt_batch_api.cpp", "lt_tune.cpp", "lt_field.hip", "lt_update.hip",
-           "lt_obs.hip", "lt_smear.hip", "lt_hmc.hip", "lt_adjoint.hip", "lt_batch.hip", "lt_io.cpp", "lt_bmw.hip"]
+           "lt_obs.hip", "lt_smear.hip", "lt_hmc.hip", "lt_adjoint.hip", "lt_batch.hip", "lt_io.cpp", "lt_bmw.hip",
+           "lt_cb4.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

@@ -37,29 +37,15 @@ __global__ void __launch_bounds__(CBS) clover_store_kernel(const double2* __rest
 // per site charge density, block partial sums
 __global__ void __launch_bounds__(CBS) qdensity_kernel(const double2* __restrict__ C, Geom g, double* part)
 {
-    __shared__ double sh[CBS];
     const int s = blockIdx.x * CBS + threadIdx.x;
-    sh[threadIdx.x] = s < g.V ? body_qdensity_site(C, g, s) : 0.0;
-    __syncthreads();
-    for (int w = CBS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    block_sum_to<CBS>(s < g.V ? body_qdensity_site(C, g, s) : 0.0, part + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(CBS) sum_kernel(const double* part, int n, double* out)
 {
-    __shared__ double sh[CBS];
     double a = 0.0;
     for (int i = threadIdx.x; i < n; i += CBS) a += part[i];
-    sh[threadIdx.x] = a;
-    __syncthreads();
-    for (int w = CBS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
+    block_sum_to<CBS>(a, out);
 }
 
 template <int HALF, int K>

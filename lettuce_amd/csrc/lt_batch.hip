@@ -109,14 +109,7 @@ __global__ void LT_LINK_KERNEL bk_sigma(const double2* __restrict__ U, double2* 
 
 __device__ __forceinline__ void block_sum(double v, double* out)
 {
-    __shared__ double sh[RBS];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = RBS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
+    block_sum_to<RBS>(v, out);
 }
 
 __global__ void __launch_bounds__(RBS) bk_copy(const double2* __restrict__ src, double2* __restrict__ dst, BatchGrid bg,

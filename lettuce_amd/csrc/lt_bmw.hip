@@ -34,22 +34,19 @@ __device__ __forceinline__ void adler_word(uint64_t w, uint64_t& r, uint64_t& a,
 }
 
 // mod-p sums of (a, b) over the block; thread 0 writes the pair
+struct OpAddMod {
+    __device__ __forceinline__ unsigned long long operator()(unsigned long long x, unsigned long long y) const
+    {
+        return (x + y) % ADLER_P;
+    }
+};
 __device__ __forceinline__ void block_sum_mod(uint64_t a, uint64_t b, uint64_t* part)
 {
-    __shared__ uint64_t sa[BS], sb[BS];
-    sa[threadIdx.x] = a % ADLER_P;
-    sb[threadIdx.x] = b % ADLER_P;
-    __syncthreads();
-    for (int w = BS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            sa[threadIdx.x] = (sa[threadIdx.x] + sa[threadIdx.x + w]) % ADLER_P;
-            sb[threadIdx.x] = (sb[threadIdx.x] + sb[threadIdx.x + w]) % ADLER_P;
-        }
-        __syncthreads();
-    }
+    unsigned long long v[2] = {a % ADLER_P, b % ADLER_P};
+    block_reduce<BS>(v, OpAddMod{});
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = sa[0];
-        part[2 * blockIdx.x + 1] = sb[0];
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
     }
 }
 
@@ -138,14 +135,9 @@ __global__ void __launch_bounds__(BS) bmw_unpack_kernel(double2* __restrict__ U,
         }
     }
     block_sum_mod(a, b, part);
-    __shared__ double sd[BS];
-    sd[threadIdx.x] = d;
-    __syncthreads();
-    for (int w = BS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) dev[blockIdx.x] = sd[0];
+    double dm[1] = {d};
+    block_reduce<BS>(dm, OpMax{});
+    if (threadIdx.x == 0) dev[blockIdx.x] = dm[0];
 }
 
 }  // namespace

@@ -519,6 +519,51 @@ __device__ __forceinline__ M3 exp_ah(const M3& A, double c)
 }
 
 // ---------------------------------------------------------------------------
+// Block reductions: a butterfly over the 64 lanes of each wavefront (cross-lane
+// exchanges, no LDS round trip and no barrier per level), then the per-wave values
+// in LDS combined in wave order by thread 0, which alone holds the result.  At each
+// butterfly level lane i forms op(v_i, v_j) and lane j forms op(v_j, v_i); op is
+// commutative, so all lanes of a wave agree and the result does not depend on the
+// scheduling.  Every thread of the block must call it; the leading barrier lets a
+// kernel call it more than once (thread 0 has read the previous call's LDS values).
+struct OpAdd {
+    template <typename T>
+    __device__ __forceinline__ T operator()(T a, T b) const { return a + b; }
+};
+struct OpMax {
+    __device__ __forceinline__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+template <int BS, typename T, int N, typename Op>
+__device__ __forceinline__ void block_reduce(T (&v)[N], Op op)
+{
+    static_assert(BS % 64 == 0 && BS <= 1024, "block of whole wavefronts");
+    constexpr int NW = BS / 64;
+    __shared__ T sh[N][NW];
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[n] = op(v[n], __shfl_xor(v[n], o, 64));
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int n = 0; n < N; ++n) sh[n][threadIdx.x >> 6] = v[n];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int n = 0; n < N; ++n)
+#pragma unroll
+            for (int w = 1; w < NW; ++w) v[n] = op(v[n], sh[n][w]);
+}
+// sum of v over the block, written to *out by thread 0
+template <int BS>
+__device__ __forceinline__ void block_sum_to(double v, double* out)
+{
+    double a[1] = {v};
+    block_reduce<BS>(a, OpAdd{});
+    if (threadIdx.x == 0) *out = a[0];
+}
+
+// ---------------------------------------------------------------------------
 // Philox4x32-10 counter RNG (Salmon et al., SC'11).  Counter layout:
 //   key = seed, ctr = {site, stream<<24 | k<<4 | sub<<2 | mu, counter_lo, counter_hi}
 enum : uint32_t { RNG_HEATBATH = 0, RNG_MOMENTUM = 1, RNG_HAAR = 2 };
@@ -577,12 +622,39 @@ __device__ __forceinline__ void gauss2(double u0, double u1, double& g0, double&
 // Lattice geometry.  Sites are lexicographic s = ((t*Nx+x)*Ny+y)*Nz+z; the parity
 // of a site is (t+x+y+z)&1 and, because Nz is even, its checkerboard index is s>>1.
 // Links live in a structure-of-arrays buffer of complex doubles:
-//   U[((mu*2 + parity)*9 + c)*Vh + cb],  c = 3*row + col,  Vh = V/2.
+//   U[((mu*2 + parity)*9 + c)*Vh + cbi(s)],  c = 3*row + col,  Vh = V/2,
+// where cbi(s) is the checkerboard index s>>1 (plain order) or, in the four-colour
+// sweep kernels, that index with each group of 16 ordered even ones first (below).
 struct Geom {
     int L[4];       // Nt, Nx, Ny, Nz
     int st[4];      // lexicographic strides of t, x, y, z
     int V, Vh;
 };
+
+// Storage slot of site s in its parity array.  Plain order: cb = s>>1.  Split order
+// (translation units compiled with LT_SPLIT_CB = 1, the four-colour sweeps): within
+// every aligned group of 16 checkerboard indices the 8 even ones come first, then the
+// 8 odd ones, so a 256-byte group of one component becomes two 128-byte lines, one
+// per cb parity (a last group of 8, when Vh is not a multiple of 16, keeps its
+// order).  With Nz divisible by 4, cb & 1 is (z>>1) & 1, so a colour class of the
+// four-colour schedule (every fourth z of a row) reads and writes whole lines instead
+// of every other 16-byte slot.  The split order costs the two-colour and Jacobi
+// kernels a few address instructions per link (measured at 32^4: RK3 flow stage
+// +13 %, observables +22 %), so they keep the plain order.  Kernels address links
+// only through this function: the order never changes a result.
+#ifndef LT_SPLIT_CB
+#define LT_SPLIT_CB 0
+#endif
+__device__ __forceinline__ int cbi(const Geom& g, int s)
+{
+    const int cb = s >> 1;
+#if LT_SPLIT_CB
+    const int p = (cb & ~15) | ((cb & 1) << 3) | ((cb >> 1) & 7);
+    return cb < (g.Vh & ~15) ? p : cb;
+#else
+    return cb;
+#endif
+}
 
 __device__ __forceinline__ void coords_of(const Geom& g, int s, int c[4])
 {
@@ -628,7 +700,7 @@ __device__ __forceinline__ double2& elem(double2* __restrict__ base, size_t i)
 
 __device__ __forceinline__ M3 ld(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
     M3 r;
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
@@ -639,7 +711,7 @@ __device__ __forceinline__ M3 ld(const double2* __restrict__ U, const Geom& g, i
 }
 __device__ __forceinline__ void st_link(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
 #pragma unroll
     for (int c = 0; c < 9; ++c) elem(U, o + (size_t)c * g.Vh) = make_double2(a.m[c].re, a.m[c].im);
 }
@@ -680,13 +752,13 @@ __device__ __forceinline__ Alg8 axpby(double a, const Alg8& x, double b, const A
 }
 __device__ __forceinline__ Alg8 ld_alg8(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
 {
-    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
     const double2 v0 = p[0], v1 = p[(size_t)g.Vh], v2 = p[(size_t)2 * g.Vh], v3 = p[(size_t)3 * g.Vh];
     return Alg8{v0.x, v0.y, cd{v1.x, v1.y}, cd{v2.x, v2.y}, cd{v3.x, v3.y}};
 }
 __device__ __forceinline__ void st_alg8(double2* __restrict__ Z, const Geom& g, int s, int mu, int par, const Alg8& z)
 {
-    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
     p[0] = make_double2(z.d0, z.d1);
     p[(size_t)g.Vh] = make_double2(z.a01.re, z.a01.im);
     p[(size_t)2 * g.Vh] = make_double2(z.a02.re, z.a02.im);
@@ -749,7 +821,7 @@ __device__ __forceinline__ void st_stream(double2* p, double2 v)
 }
 __device__ __forceinline__ Alg8 ld_alg8_stream(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
 {
-    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
     const double2 v0 = ld_stream(p), v1 = ld_stream(p + (size_t)g.Vh), v2 = ld_stream(p + (size_t)2 * g.Vh),
                   v3 = ld_stream(p + (size_t)3 * g.Vh);
     return Alg8{v0.x, v0.y, cd{v1.x, v1.y}, cd{v2.x, v2.y}, cd{v3.x, v3.y}};
@@ -757,7 +829,7 @@ __device__ __forceinline__ Alg8 ld_alg8_stream(const double2* __restrict__ Z, co
 __device__ __forceinline__ void st_alg8_stream(double2* __restrict__ Z, const Geom& g, int s, int mu, int par,
                                                const Alg8& z)
 {
-    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
     st_stream(p, make_double2(z.d0, z.d1));
     st_stream(p + (size_t)g.Vh, make_double2(z.a01.re, z.a01.im));
     st_stream(p + (size_t)2 * g.Vh, make_double2(z.a02.re, z.a02.im));
@@ -770,7 +842,7 @@ __device__ __forceinline__ void st_alg8_stream(double2* __restrict__ Z, const Ge
 // Im q_ii = 0, q_22 = -(q_00 + q_11) and q_ji = conj(q_ij) are implied.
 __device__ __forceinline__ M3 ld_her8(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
 {
-    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
     const double2 v0 = p[0], v1 = p[(size_t)g.Vh], v2 = p[(size_t)2 * g.Vh], v3 = p[(size_t)3 * g.Vh];
     M3 r;
     r.m[0] = cd{v0.x, 0.0};
@@ -786,7 +858,7 @@ __device__ __forceinline__ M3 ld_her8(const double2* __restrict__ Z, const Geom&
 }
 __device__ __forceinline__ void st_her8(double2* __restrict__ Z, const Geom& g, int s, int mu, int par, const M3& q)
 {
-    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + (s >> 1);
+    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
     p[0] = make_double2(q.m[0].re, q.m[4].re);
     p[(size_t)g.Vh] = make_double2(q.m[1].re, q.m[1].im);
     p[(size_t)2 * g.Vh] = make_double2(q.m[2].re, q.m[2].im);
@@ -845,7 +917,7 @@ __device__ __forceinline__ void acc_su3_r01(M3& acc, const M3& t)
 }
 __device__ __forceinline__ M3 ldu(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
     M3 r;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -858,7 +930,7 @@ __device__ __forceinline__ M3 ldu(const double2* __restrict__ U, const Geom& g, 
 // rows 0 and 1 of an SU(3) link; row 2 is left zero (callers never read it)
 __device__ __forceinline__ M3 ldu2(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
     M3 r;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -870,14 +942,14 @@ __device__ __forceinline__ M3 ldu2(const double2* __restrict__ U, const Geom& g,
 }
 __device__ __forceinline__ void stu(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
 #pragma unroll
     for (int c = 0; c < 6; ++c) elem(U, o + (size_t)c * g.Vh) = make_double2(a.m[c].re, a.m[c].im);
 }
 
 __device__ __forceinline__ void stu_stream(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
 {
-    double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
 #pragma unroll
     for (int c = 0; c < 6; ++c) st_stream(p + (size_t)c * g.Vh, make_double2(a.m[c].re, a.m[c].im));
 }
@@ -1014,7 +1086,7 @@ struct Raw6 {
 };
 __device__ __forceinline__ void ld_raw(const double2* __restrict__ U, const Geom& g, int s, int mu, int par, Raw6& r)
 {
-    const double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + (s >> 1);
+    const double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
 #pragma unroll
     for (int c = 0; c < 6; ++c) r.v[c] = p[(size_t)c * g.Vh];
 }

@@ -87,8 +87,6 @@ __global__ void __launch_bounds__(BS) reunit_kernel(double2* __restrict__ U, Geo
 // per-block max |U^+U - 1| (Frobenius, per link) and count of non-finite entries
 __global__ void __launch_bounds__(BS) check_kernel(const double2* __restrict__ U, Geom g, int su3, double* part)
 {
-    __shared__ double smax[BS];
-    __shared__ double scnt[BS];
     int cb = blockIdx.x * BS + threadIdx.x;
     double dev = 0.0, bad = 0.0;
     if (cb < g.Vh) {
@@ -109,20 +107,13 @@ __global__ void __launch_bounds__(BS) check_kernel(const double2* __restrict__ U
         dev = sqrt(f);
         if (!(dev == dev)) dev = 1.0e300;
     }
-    smax[threadIdx.x] = dev;
-    scnt[threadIdx.x] = bad;
-    __syncthreads();
-    for (int w = BS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
-            scnt[threadIdx.x] += scnt[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
+    double m[1] = {dev}, n[1] = {bad};
+    block_reduce<BS>(m, OpMax{});
+    block_reduce<BS>(n, OpAdd{});
     if (threadIdx.x == 0) {
         size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-        part[2 * b] = smax[0];
-        part[2 * b + 1] = scnt[0];
+        part[2 * b] = m[0];
+        part[2 * b + 1] = n[0];
     }
 }
 

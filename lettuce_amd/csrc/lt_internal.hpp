@@ -235,6 +235,15 @@ void k_check(lt_field f, double* max_dev, long long* nonfinite);
 void k_staple_field(lt_field u, lt_field out);
 void k_or_sweep(lt_field u, int n, int schedule);
 void k_hb_sweep(lt_field u, double beta, int n, int schedule);
+// four-colour sweeps (lt_cb4.hip): the field's rows 0, 1 reordered in place between the
+// plain cb order and the split order (to_split 1 / 0), and single colour launches that
+// expect the split order
+void k_cb_layout(lt_field u, int to_split);
+void k_cb4_colour(lt_field u, bool heatbath, int mu, int colour, double prefactor, uint64_t counter, const char* name);
+void k_cb4_colour_graph_hb(lt_field u, int mu, int colour, double prefactor, const uint64_t* state, int k);
+void k_cb4_colour_action(lt_field u, bool heatbath, int mu, int colour, double prefactor, uint64_t counter,
+                         double c_plaq, double c_rect, const char* name);
+void k_cb4_plaq_partials(lt_field u, double* part, unsigned nb);
 // n_update x (n_hb heatbath + n_or overrelaxation sweeps) as one graph replayed per
 // update, PlaquetteSum after each into plaq_host (may be null); and its teardown
 void k_update_graph(lt_field u, double beta, int n_hb, int n_or, int schedule, int n_update, double* plaq_host);

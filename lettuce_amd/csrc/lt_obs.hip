@@ -40,23 +40,12 @@ __device__ __forceinline__ double energy_term(const M3& F)
 
 __device__ __forceinline__ void block_sum3(double v0, double v1, double v2, double* out)
 {
-    __shared__ double sh[3][BS];
-    sh[0][threadIdx.x] = v0;
-    sh[1][threadIdx.x] = v1;
-    sh[2][threadIdx.x] = v2;
-    __syncthreads();
-    for (int w = BS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
-            sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
-        }
-        __syncthreads();
-    }
+    double a[3] = {v0, v1, v2};
+    block_reduce<BS>(a, OpAdd{});
     if (threadIdx.x == 0) {
-        out[0] = sh[0][0];
-        out[1] = sh[1][0];
-        out[2] = sh[2][0];
+        out[0] = a[0];
+        out[1] = a[1];
+        out[2] = a[2];
     }
 }
 
@@ -199,20 +188,11 @@ __device__ __forceinline__ void plane_loops(const double2* __restrict__ U, const
 
 __device__ __forceinline__ void block_sum2(double v0, double v1, double* out)
 {
-    __shared__ double sh[2][BS];
-    sh[0][threadIdx.x] = v0;
-    sh[1][threadIdx.x] = v1;
-    __syncthreads();
-    for (int w = BS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
-        }
-        __syncthreads();
-    }
+    double a[2] = {v0, v1};
+    block_reduce<BS>(a, OpAdd{});
     if (threadIdx.x == 0) {
-        out[0] = sh[0][0];
-        out[1] = sh[1][0];
+        out[0] = a[0];
+        out[1] = a[1];
     }
 }
 
