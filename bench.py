@@ -177,6 +177,44 @@ def pt_metad(la, ctx, shapes, n_step=20, steps=2, graph=True):
             "batched_site_traj_per_s": sites / t_batch, "accept_rate_last_step": float(res["accepted"].mean())}
 
 
+def metad16(la, ctx, n_traj=3, n_step=20):
+    """The 16^4 metadynamics configuration: MetaD-HMC trajectories (OMF2 tau = 1, n_step
+    steps) with the bias force through clover Q after 4 stout levels rho = 0.12, from a
+    thermalised start; seconds per trajectory, per force evaluation (2 n_step + 1 per
+    trajectory), and the kernel time of each force kernel per evaluation."""
+    shape = (16,) * 4
+    U = la.GaugeField(ctx, shape)
+    U.upload(thermalised(la, ctx, shape, 0, 20))
+    U.set_rng(0, 0)
+    Uc, P, F = la.GaugeField(ctx, shape), la.MatrixField(ctx, shape), la.MatrixField(ctx, shape)
+    cv = la.CollectiveVariable(ctx, shape, 4, 0.12)
+    k = la.HMCMetaDKernel(U, Uc, P, F, c3_bias(la), cv, "omf2", 6.0, 1.0, update_bias=True)
+    k(n_step)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_traj):
+        k(n_step)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / n_traj
+    ctx.profile(True)
+    k(n_step)
+    ctx.synchronize()
+    n_eval = 2 * n_step + 1
+    kernels = {}
+    for name in ("stout_tape", "clover_store", "qdensity", "clover_deriv", "stout_lambda", "stout_sigma_fs",
+                 "stout_sigma", "force_update", "field_update"):
+        n, ms = ctx.kernel_time(name)
+        if n:
+            kernels[name] = {"launches_per_eval": n / n_eval, "us_per_eval": 1e3 * ms / n_eval}
+    ctx.profile(False)
+    cv.close()
+    for f in (U, Uc, P, F):
+        f.close()
+    return {"lattice": list(shape), "config": f"OMF2 tau=1 n_step={n_step}, 4 stout levels rho=0.12",
+            "traj_s": dt, "traj_per_s": 1.0 / dt, "force_evals_per_traj": n_eval,
+            "ms_per_force_eval": 1e3 * dt / n_eval, "kernels": kernels}
+
+
 def lattice8(la, ctx, n_update=200):
     """8^4 at beta = 5.7: n_update HB + 4 OR updates from Haar(0), plaquette after each (the
     correctness configuration: launch-latency bound).  Timed twice from the same start:
@@ -290,6 +328,8 @@ def main():
     ap.add_argument("--no-48", action="store_true", help="skip the 48^4 measurement")
     ap.add_argument("--pt-16-only", action="store_true",
                     help="of the PT-MetaD measurements run only 16 x 16^4 (kernel profiles of that workload)")
+    ap.add_argument("--metad-only", action="store_true",
+                    help="run only the 16^4 MetaD-HMC trajectories and print their line (kernel counter runs)")
     ap.add_argument("--eager", action="store_true",
                     help="PT-MetaD steps as eager launches instead of graph replays (kernel tracing)")
     args = ap.parse_args()
@@ -298,6 +338,13 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.metad_only:
+        import lettuce_amd as la
+        ctx = la.Context(0)
+        print(json.dumps({"metad_16": metad16(la, ctx)}))
+        ctx.close()
+        return
 
     # the reference's own code runs as a child process before this process touches the GPU
     ref_cpu = None
@@ -477,6 +524,8 @@ def main():
     U.close()
     print("# 8^4 correctness configuration", file=sys.stderr, flush=True)
     result["lattice_8"] = lattice8(la, ctx)
+    print("# MetaD-HMC 16^4", file=sys.stderr, flush=True)
+    result["metad_16"] = metad16(la, ctx)
     if not args.no_pt:
         print("# PT-MetaD 16 x 16^4", file=sys.stderr, flush=True)
         g16 = (16, 16, 16, 16)

@@ -122,9 +122,9 @@ __device__ __forceinline__ int plane_of(int a, int b)
 // Re r_ii = 0 and r_ji = -conj(r_ij) hold exactly for C - C^+ in floating point.
 __device__ __forceinline__ M3 ld_plane(const double2* __restrict__ R, const Geom& g, int s, int plane, int par)
 {
-    const double2* p = R + (size_t)((plane * 2 + par) * R_SLOTS) * g.Vh + cbi(g, s);
-    const double2 a01 = p[0], a02 = p[(size_t)g.Vh], a12 = p[(size_t)2 * g.Vh];
-    const double2 d01 = p[(size_t)3 * g.Vh], d2 = p[(size_t)4 * g.Vh];
+    const double2* p = R + slot_base(g, plane * 2 + par, R_SLOTS, cbi(g, s));
+    const double2 a01 = p[0], a02 = p[slot_stride(g)], a12 = p[(size_t)2 * slot_stride(g)];
+    const double2 d01 = p[(size_t)3 * slot_stride(g)], d2 = p[(size_t)4 * slot_stride(g)];
     M3 r;
     r.m[0] = cd{0.0, d01.x};
     r.m[4] = cd{0.0, d01.y};
@@ -140,13 +140,13 @@ __device__ __forceinline__ M3 ld_plane(const double2* __restrict__ R, const Geom
 // stores R = C - C^+ of the clover term C
 __device__ __forceinline__ void st_plane(double2* __restrict__ R, const Geom& g, int s, int plane, int par, const M3& c)
 {
-    double2* p = R + (size_t)((plane * 2 + par) * R_SLOTS) * g.Vh + cbi(g, s);
+    double2* p = R + slot_base(g, plane * 2 + par, R_SLOTS, cbi(g, s));
     const cd r01 = c.m[1] - conj(c.m[3]), r02 = c.m[2] - conj(c.m[6]), r12 = c.m[5] - conj(c.m[7]);
     p[0] = make_double2(r01.re, r01.im);
-    p[(size_t)g.Vh] = make_double2(r02.re, r02.im);
-    p[(size_t)2 * g.Vh] = make_double2(r12.re, r12.im);
-    p[(size_t)3 * g.Vh] = make_double2(c.m[0].im + c.m[0].im, c.m[4].im + c.m[4].im);
-    p[(size_t)4 * g.Vh] = make_double2(c.m[8].im + c.m[8].im, 0.0);
+    p[slot_stride(g)] = make_double2(r02.re, r02.im);
+    p[(size_t)2 * slot_stride(g)] = make_double2(r12.re, r12.im);
+    p[(size_t)3 * slot_stride(g)] = make_double2(c.m[0].im + c.m[0].im, c.m[4].im + c.m[4].im);
+    p[(size_t)4 * slot_stride(g)] = make_double2(c.m[8].im + c.m[8].im, 0.0);
 }
 
 // R_rhosigma(y) = C_rhosigma - C_sigmarho with C_sigmarho = C_rhosigma^+

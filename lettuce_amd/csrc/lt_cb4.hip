@@ -72,13 +72,13 @@ __global__ void __launch_bounds__(RBS) cb_layout_kernel(double2* __restrict__ U,
     const long long id = (long long)blockIdx.x * RBS + threadIdx.x;
     if (id >= 48 * vh16) return;
     const int a = (int)(id / vh16);            // (mu*2 + parity)*6 + component
-    const long long e = id - a * vh16;
-    double2* p = U + (size_t)((a / 6) * 9 + a % 6) * g.Vh + (e & ~15LL);
-    const int l = (int)(e & 15);
+    const int e = (int)(id - a * vh16);
+    const int grp = e & ~15, l = e & 15;
     const int dst = to_split ? ((l & 1) << 3) | (l >> 1)    // cb l -> slot
                              : (l < 8 ? 2 * l : 2 * (l - 8) + 1);   // slot l -> cb
-    const double2 v = p[l];
-    p[dst] = v;
+    double2* p = U + (size_t)(a % 6) * slot_stride(g);
+    const double2 v = p[slot_base(g, a / 6, 9, grp + l)];
+    p[slot_base(g, a / 6, 9, grp + dst)] = v;
 }
 
 }  // namespace

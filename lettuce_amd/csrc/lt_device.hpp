@@ -113,11 +113,29 @@ __device__ __forceinline__ M3 m3_id()
 // sum_k op(a_k) op(b_k) over three complex pairs as two chains of one multiply
 // and five fused multiply-adds (op = conj where CA / CB is set): 12 FP64
 // instructions per complex entry instead of 16 for separate products and sums.
+#ifndef LT_DOT_SPLIT
+#define LT_DOT_SPLIT 0
+#endif
 template <bool CA, bool CB>
 __device__ __forceinline__ cd dot3(cd a0, cd b0, cd a1, cd b1, cd a2, cd b2)
 {
     // (x.re + i sx x.im)(y.re + i sy y.im): re = x.re y.re - sx sy x.im y.im, im = sy x.re y.im + sx x.im y.re
     constexpr double sa = CA ? -1.0 : 1.0, sb = CB ? -1.0 : 1.0, sab = -sa * sb;
+#if LT_DOT_SPLIT
+    double re = a0.re * b0.re;
+    re = fma(sab * a0.im, b0.im, re);
+    re = fma(a1.re, b1.re, re);
+    double re2 = (sab * a1.im) * b1.im;
+    re2 = fma(a2.re, b2.re, re2);
+    re2 = fma(sab * a2.im, b2.im, re2);
+    double im = (sb * a0.re) * b0.im;
+    im = fma(sa * a0.im, b0.re, im);
+    im = fma(sb * a1.re, b1.im, im);
+    double im2 = (sa * a1.im) * b1.re;
+    im2 = fma(sb * a2.re, b2.im, im2);
+    im2 = fma(sa * a2.im, b2.re, im2);
+    return cd{re + re2, im + im2};
+#else
     double re = a0.re * b0.re;
     re = fma(sab * a0.im, b0.im, re);
     re = fma(a1.re, b1.re, re);
@@ -131,12 +149,28 @@ __device__ __forceinline__ cd dot3(cd a0, cd b0, cd a1, cd b1, cd a2, cd b2)
     im = fma(sb * a2.re, b2.im, im);
     im = fma(sa * a2.im, b2.re, im);
     return cd{re, im};
+#endif
 }
 // acc + sum_k op(a_k) op(b_k): the same chains started from acc (6 FMAs per part)
 template <bool CA, bool CB>
 __device__ __forceinline__ cd dot3_acc(cd acc, cd a0, cd b0, cd a1, cd b1, cd a2, cd b2)
 {
     constexpr double sa = CA ? -1.0 : 1.0, sb = CB ? -1.0 : 1.0, sab = -sa * sb;
+#if LT_DOT_SPLIT
+    double re = fma(a0.re, b0.re, acc.re);
+    re = fma(sab * a0.im, b0.im, re);
+    re = fma(a1.re, b1.re, re);
+    double re2 = (sab * a1.im) * b1.im;
+    re2 = fma(a2.re, b2.re, re2);
+    re2 = fma(sab * a2.im, b2.im, re2);
+    double im = fma(sb * a0.re, b0.im, acc.im);
+    im = fma(sa * a0.im, b0.re, im);
+    im = fma(sb * a1.re, b1.im, im);
+    double im2 = (sa * a1.im) * b1.re;
+    im2 = fma(sb * a2.re, b2.im, im2);
+    im2 = fma(sa * a2.im, b2.re, im2);
+    return cd{re + re2, im + im2};
+#else
     double re = fma(a0.re, b0.re, acc.re);
     re = fma(sab * a0.im, b0.im, re);
     re = fma(a1.re, b1.re, re);
@@ -150,6 +184,7 @@ __device__ __forceinline__ cd dot3_acc(cd acc, cd a0, cd b0, cd a1, cd b1, cd a2
     im = fma(sb * a2.re, b2.im, im);
     im = fma(sa * a2.im, b2.re, im);
     return cd{re, im};
+#endif
 }
 // acc += a * b
 __device__ __forceinline__ void mul_acc(M3& acc, const M3& a, const M3& b)
@@ -656,6 +691,33 @@ __device__ __forceinline__ int cbi(const Geom& g, int s)
 #endif
 }
 
+// Offset, in complex doubles, of slot 0 of storage position pos of array a (a =
+// mu*2 + parity, or plane*2 + parity) in a buffer of nslot slots per entry, and the
+// distance between consecutive slots of one entry.  Plain: one array of Vh entries
+// per slot.  LT_AOSOA: blocks of 8 consecutive positions, each block holding its
+// nslot slots one after another (one 128-byte line per slot), so the slots of an
+// entry lie within nslot lines.
+#ifndef LT_AOSOA
+#define LT_AOSOA 0
+#endif
+constexpr int AOS_B = 8;
+__device__ __forceinline__ size_t slot_base(const Geom& g, int a, int nslot, int pos)
+{
+#if LT_AOSOA
+    return ((size_t)a * (size_t)(g.Vh / AOS_B) + (size_t)(pos / AOS_B)) * (size_t)(nslot * AOS_B) + (size_t)(pos % AOS_B);
+#else
+    return (size_t)a * (size_t)nslot * (size_t)g.Vh + (size_t)pos;
+#endif
+}
+__device__ __forceinline__ size_t slot_stride(const Geom& g)
+{
+#if LT_AOSOA
+    return AOS_B;
+#else
+    return (size_t)g.Vh;
+#endif
+}
+
 __device__ __forceinline__ void coords_of(const Geom& g, int s, int c[4])
 {
     c[3] = s % g.L[3];
@@ -700,20 +762,20 @@ __device__ __forceinline__ double2& elem(double2* __restrict__ base, size_t i)
 
 __device__ __forceinline__ M3 ld(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
+    const size_t o = slot_base(g, mu * 2 + par, 9, cbi(g, s));
     M3 r;
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-        double2 v = elem(U, o + (size_t)c * g.Vh);
+        double2 v = elem(U, o + (size_t)c * slot_stride(g));
         r.m[c] = cd{v.x, v.y};
     }
     return r;
 }
 __device__ __forceinline__ void st_link(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
+    const size_t o = slot_base(g, mu * 2 + par, 9, cbi(g, s));
 #pragma unroll
-    for (int c = 0; c < 9; ++c) elem(U, o + (size_t)c * g.Vh) = make_double2(a.m[c].re, a.m[c].im);
+    for (int c = 0; c < 9; ++c) elem(U, o + (size_t)c * slot_stride(g)) = make_double2(a.m[c].re, a.m[c].im);
 }
 // su(3) algebra fields (anti-hermitian, traceless: the Wilson-flow Z) in 8 reals,
 //   Z[((mu*2 + parity)*4 + c)*Vh + cb],  c: 0 = (Im a00, Im a11), 1 = a01, 2 = a02, 3 = a12;
@@ -752,17 +814,17 @@ __device__ __forceinline__ Alg8 axpby(double a, const Alg8& x, double b, const A
 }
 __device__ __forceinline__ Alg8 ld_alg8(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
 {
-    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
-    const double2 v0 = p[0], v1 = p[(size_t)g.Vh], v2 = p[(size_t)2 * g.Vh], v3 = p[(size_t)3 * g.Vh];
+    const double2* p = Z + slot_base(g, mu * 2 + par, 4, cbi(g, s));
+    const double2 v0 = p[0], v1 = p[slot_stride(g)], v2 = p[(size_t)2 * slot_stride(g)], v3 = p[(size_t)3 * slot_stride(g)];
     return Alg8{v0.x, v0.y, cd{v1.x, v1.y}, cd{v2.x, v2.y}, cd{v3.x, v3.y}};
 }
 __device__ __forceinline__ void st_alg8(double2* __restrict__ Z, const Geom& g, int s, int mu, int par, const Alg8& z)
 {
-    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
+    double2* p = Z + slot_base(g, mu * 2 + par, 4, cbi(g, s));
     p[0] = make_double2(z.d0, z.d1);
-    p[(size_t)g.Vh] = make_double2(z.a01.re, z.a01.im);
-    p[(size_t)2 * g.Vh] = make_double2(z.a02.re, z.a02.im);
-    p[(size_t)3 * g.Vh] = make_double2(z.a12.re, z.a12.im);
+    p[slot_stride(g)] = make_double2(z.a01.re, z.a01.im);
+    p[(size_t)2 * slot_stride(g)] = make_double2(z.a02.re, z.a02.im);
+    p[(size_t)3 * slot_stride(g)] = make_double2(z.a12.re, z.a12.im);
 }
 
 // exp(c Z) for Z anti-hermitian traceless in the 8-real form (the flow's Z):
@@ -821,19 +883,19 @@ __device__ __forceinline__ void st_stream(double2* p, double2 v)
 }
 __device__ __forceinline__ Alg8 ld_alg8_stream(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
 {
-    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
-    const double2 v0 = ld_stream(p), v1 = ld_stream(p + (size_t)g.Vh), v2 = ld_stream(p + (size_t)2 * g.Vh),
-                  v3 = ld_stream(p + (size_t)3 * g.Vh);
+    const double2* p = Z + slot_base(g, mu * 2 + par, 4, cbi(g, s));
+    const double2 v0 = ld_stream(p), v1 = ld_stream(p + slot_stride(g)), v2 = ld_stream(p + (size_t)2 * slot_stride(g)),
+                  v3 = ld_stream(p + (size_t)3 * slot_stride(g));
     return Alg8{v0.x, v0.y, cd{v1.x, v1.y}, cd{v2.x, v2.y}, cd{v3.x, v3.y}};
 }
 __device__ __forceinline__ void st_alg8_stream(double2* __restrict__ Z, const Geom& g, int s, int mu, int par,
                                                const Alg8& z)
 {
-    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
+    double2* p = Z + slot_base(g, mu * 2 + par, 4, cbi(g, s));
     st_stream(p, make_double2(z.d0, z.d1));
-    st_stream(p + (size_t)g.Vh, make_double2(z.a01.re, z.a01.im));
-    st_stream(p + (size_t)2 * g.Vh, make_double2(z.a02.re, z.a02.im));
-    st_stream(p + (size_t)3 * g.Vh, make_double2(z.a12.re, z.a12.im));
+    st_stream(p + slot_stride(g), make_double2(z.a01.re, z.a01.im));
+    st_stream(p + (size_t)2 * slot_stride(g), make_double2(z.a02.re, z.a02.im));
+    st_stream(p + (size_t)3 * slot_stride(g), make_double2(z.a12.re, z.a12.im));
 }
 
 // Traceless hermitian fields (the stout exponents Q of the CV tape, the stout-force
@@ -842,8 +904,8 @@ __device__ __forceinline__ void st_alg8_stream(double2* __restrict__ Z, const Ge
 // Im q_ii = 0, q_22 = -(q_00 + q_11) and q_ji = conj(q_ij) are implied.
 __device__ __forceinline__ M3 ld_her8(const double2* __restrict__ Z, const Geom& g, int s, int mu, int par)
 {
-    const double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
-    const double2 v0 = p[0], v1 = p[(size_t)g.Vh], v2 = p[(size_t)2 * g.Vh], v3 = p[(size_t)3 * g.Vh];
+    const double2* p = Z + slot_base(g, mu * 2 + par, 4, cbi(g, s));
+    const double2 v0 = p[0], v1 = p[slot_stride(g)], v2 = p[(size_t)2 * slot_stride(g)], v3 = p[(size_t)3 * slot_stride(g)];
     M3 r;
     r.m[0] = cd{v0.x, 0.0};
     r.m[4] = cd{v0.y, 0.0};
@@ -858,11 +920,11 @@ __device__ __forceinline__ M3 ld_her8(const double2* __restrict__ Z, const Geom&
 }
 __device__ __forceinline__ void st_her8(double2* __restrict__ Z, const Geom& g, int s, int mu, int par, const M3& q)
 {
-    double2* p = Z + (size_t)((mu * 2 + par) * 4) * g.Vh + cbi(g, s);
+    double2* p = Z + slot_base(g, mu * 2 + par, 4, cbi(g, s));
     p[0] = make_double2(q.m[0].re, q.m[4].re);
-    p[(size_t)g.Vh] = make_double2(q.m[1].re, q.m[1].im);
-    p[(size_t)2 * g.Vh] = make_double2(q.m[2].re, q.m[2].im);
-    p[(size_t)3 * g.Vh] = make_double2(q.m[5].re, q.m[5].im);
+    p[slot_stride(g)] = make_double2(q.m[1].re, q.m[1].im);
+    p[(size_t)2 * slot_stride(g)] = make_double2(q.m[2].re, q.m[2].im);
+    p[(size_t)3 * slot_stride(g)] = make_double2(q.m[5].re, q.m[5].im);
 }
 // the matrix that st_her8 followed by ld_her8 reproduces
 __device__ __forceinline__ M3 her8_round(const M3& q)
@@ -917,11 +979,11 @@ __device__ __forceinline__ void acc_su3_r01(M3& acc, const M3& t)
 }
 __device__ __forceinline__ M3 ldu(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
+    const size_t o = slot_base(g, mu * 2 + par, 9, cbi(g, s));
     M3 r;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-        double2 v = elem(U, o + (size_t)c * g.Vh);
+        double2 v = elem(U, o + (size_t)c * slot_stride(g));
         r.m[c] = cd{v.x, v.y};
     }
     third_row(r);
@@ -930,11 +992,11 @@ __device__ __forceinline__ M3 ldu(const double2* __restrict__ U, const Geom& g, 
 // rows 0 and 1 of an SU(3) link; row 2 is left zero (callers never read it)
 __device__ __forceinline__ M3 ldu2(const double2* __restrict__ U, const Geom& g, int s, int mu, int par)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
+    const size_t o = slot_base(g, mu * 2 + par, 9, cbi(g, s));
     M3 r;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-        double2 v = elem(U, o + (size_t)c * g.Vh);
+        double2 v = elem(U, o + (size_t)c * slot_stride(g));
         r.m[c] = cd{v.x, v.y};
     }
     r.m[6] = r.m[7] = r.m[8] = cd{0.0, 0.0};
@@ -942,16 +1004,16 @@ __device__ __forceinline__ M3 ldu2(const double2* __restrict__ U, const Geom& g,
 }
 __device__ __forceinline__ void stu(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
 {
-    const size_t o = (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
+    const size_t o = slot_base(g, mu * 2 + par, 9, cbi(g, s));
 #pragma unroll
-    for (int c = 0; c < 6; ++c) elem(U, o + (size_t)c * g.Vh) = make_double2(a.m[c].re, a.m[c].im);
+    for (int c = 0; c < 6; ++c) elem(U, o + (size_t)c * slot_stride(g)) = make_double2(a.m[c].re, a.m[c].im);
 }
 
 __device__ __forceinline__ void stu_stream(double2* __restrict__ U, const Geom& g, int s, int mu, int par, const M3& a)
 {
-    double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
+    double2* p = U + slot_base(g, mu * 2 + par, 9, cbi(g, s));
 #pragma unroll
-    for (int c = 0; c < 6; ++c) st_stream(p + (size_t)c * g.Vh, make_double2(a.m[c].re, a.m[c].im));
+    for (int c = 0; c < 6; ++c) st_stream(p + (size_t)c * slot_stride(g), make_double2(a.m[c].re, a.m[c].im));
 }
 
 // Clover planes R_munu = C_munu - C_munu^+ (mu < nu) are stored anti-hermitian in
@@ -1086,9 +1148,9 @@ struct Raw6 {
 };
 __device__ __forceinline__ void ld_raw(const double2* __restrict__ U, const Geom& g, int s, int mu, int par, Raw6& r)
 {
-    const double2* p = U + (size_t)((mu * 2 + par) * 9) * g.Vh + cbi(g, s);
+    const double2* p = U + slot_base(g, mu * 2 + par, 9, cbi(g, s));
 #pragma unroll
-    for (int c = 0; c < 6; ++c) r.v[c] = p[(size_t)c * g.Vh];
+    for (int c = 0; c < 6; ++c) r.v[c] = p[(size_t)c * slot_stride(g)];
 }
 __device__ __forceinline__ M3 rows01(const Raw6& r)
 {

@@ -46,9 +46,9 @@ __global__ void __launch_bounds__(BS) identity_kernel(double2* __restrict__ U, G
     int cb = blockIdx.x * BS + threadIdx.x;
     if (cb >= g.Vh) return;
     int mp = blockIdx.y;   // mu*2 + parity
-    double2* p = U + (size_t)mp * 9 * g.Vh + cb;
+    double2* p = U + slot_base(g, mp, 9, cb);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) p[(size_t)c * g.Vh] = make_double2((c % 4 == 0) ? 1.0 : 0.0, 0.0);
+    for (int c = 0; c < 9; ++c) p[(size_t)c * slot_stride(g)] = make_double2((c % 4 == 0) ? 1.0 : 0.0, 0.0);
 }
 
 // Haar SU(3): complex Gaussian rows 0 and 1 (Philox stream RNG_HAAR, draw block k
