@@ -113,29 +113,11 @@ __device__ __forceinline__ M3 m3_id()
 // sum_k op(a_k) op(b_k) over three complex pairs as two chains of one multiply
 // and five fused multiply-adds (op = conj where CA / CB is set): 12 FP64
 // instructions per complex entry instead of 16 for separate products and sums.
-#ifndef LT_DOT_SPLIT
-#define LT_DOT_SPLIT 0
-#endif
 template <bool CA, bool CB>
 __device__ __forceinline__ cd dot3(cd a0, cd b0, cd a1, cd b1, cd a2, cd b2)
 {
     // (x.re + i sx x.im)(y.re + i sy y.im): re = x.re y.re - sx sy x.im y.im, im = sy x.re y.im + sx x.im y.re
     constexpr double sa = CA ? -1.0 : 1.0, sb = CB ? -1.0 : 1.0, sab = -sa * sb;
-#if LT_DOT_SPLIT
-    double re = a0.re * b0.re;
-    re = fma(sab * a0.im, b0.im, re);
-    re = fma(a1.re, b1.re, re);
-    double re2 = (sab * a1.im) * b1.im;
-    re2 = fma(a2.re, b2.re, re2);
-    re2 = fma(sab * a2.im, b2.im, re2);
-    double im = (sb * a0.re) * b0.im;
-    im = fma(sa * a0.im, b0.re, im);
-    im = fma(sb * a1.re, b1.im, im);
-    double im2 = (sa * a1.im) * b1.re;
-    im2 = fma(sb * a2.re, b2.im, im2);
-    im2 = fma(sa * a2.im, b2.re, im2);
-    return cd{re + re2, im + im2};
-#else
     double re = a0.re * b0.re;
     re = fma(sab * a0.im, b0.im, re);
     re = fma(a1.re, b1.re, re);
@@ -149,28 +131,12 @@ __device__ __forceinline__ cd dot3(cd a0, cd b0, cd a1, cd b1, cd a2, cd b2)
     im = fma(sb * a2.re, b2.im, im);
     im = fma(sa * a2.im, b2.re, im);
     return cd{re, im};
-#endif
 }
 // acc + sum_k op(a_k) op(b_k): the same chains started from acc (6 FMAs per part)
 template <bool CA, bool CB>
 __device__ __forceinline__ cd dot3_acc(cd acc, cd a0, cd b0, cd a1, cd b1, cd a2, cd b2)
 {
     constexpr double sa = CA ? -1.0 : 1.0, sb = CB ? -1.0 : 1.0, sab = -sa * sb;
-#if LT_DOT_SPLIT
-    double re = fma(a0.re, b0.re, acc.re);
-    re = fma(sab * a0.im, b0.im, re);
-    re = fma(a1.re, b1.re, re);
-    double re2 = (sab * a1.im) * b1.im;
-    re2 = fma(a2.re, b2.re, re2);
-    re2 = fma(sab * a2.im, b2.im, re2);
-    double im = fma(sb * a0.re, b0.im, acc.im);
-    im = fma(sa * a0.im, b0.re, im);
-    im = fma(sb * a1.re, b1.im, im);
-    double im2 = (sa * a1.im) * b1.re;
-    im2 = fma(sb * a2.re, b2.im, im2);
-    im2 = fma(sa * a2.im, b2.re, im2);
-    return cd{re + re2, im + im2};
-#else
     double re = fma(a0.re, b0.re, acc.re);
     re = fma(sab * a0.im, b0.im, re);
     re = fma(a1.re, b1.re, re);
@@ -184,7 +150,6 @@ __device__ __forceinline__ cd dot3_acc(cd acc, cd a0, cd b0, cd a1, cd b1, cd a2
     im = fma(sb * a2.re, b2.im, im);
     im = fma(sa * a2.im, b2.re, im);
     return cd{re, im};
-#endif
 }
 // acc += a * b
 __device__ __forceinline__ void mul_acc(M3& acc, const M3& a, const M3& b)
@@ -693,30 +658,14 @@ __device__ __forceinline__ int cbi(const Geom& g, int s)
 
 // Offset, in complex doubles, of slot 0 of storage position pos of array a (a =
 // mu*2 + parity, or plane*2 + parity) in a buffer of nslot slots per entry, and the
-// distance between consecutive slots of one entry.  Plain: one array of Vh entries
-// per slot.  LT_AOSOA: blocks of 8 consecutive positions, each block holding its
-// nslot slots one after another (one 128-byte line per slot), so the slots of an
-// entry lie within nslot lines.
-#ifndef LT_AOSOA
-#define LT_AOSOA 0
-#endif
-constexpr int AOS_B = 8;
+// distance between consecutive slots of one entry: one array of Vh entries per slot.
+// (Blocks of 8 positions holding their slots one after another were measured at 32^4:
+// overrelaxation +3 %, heatbath -3 %, flow unchanged; not kept.)
 __device__ __forceinline__ size_t slot_base(const Geom& g, int a, int nslot, int pos)
 {
-#if LT_AOSOA
-    return ((size_t)a * (size_t)(g.Vh / AOS_B) + (size_t)(pos / AOS_B)) * (size_t)(nslot * AOS_B) + (size_t)(pos % AOS_B);
-#else
     return (size_t)a * (size_t)nslot * (size_t)g.Vh + (size_t)pos;
-#endif
 }
-__device__ __forceinline__ size_t slot_stride(const Geom& g)
-{
-#if LT_AOSOA
-    return AOS_B;
-#else
-    return (size_t)g.Vh;
-#endif
-}
+__device__ __forceinline__ size_t slot_stride(const Geom& g) { return (size_t)g.Vh; }
 
 __device__ __forceinline__ void coords_of(const Geom& g, int s, int c[4])
 {
