@@ -24,12 +24,14 @@ constexpr int BS = LT_LINK_BS;
 constexpr int RBS = 256;
 
 // One thread per link of the class (cb4_site), consecutive threads along z.
+// Colours 1 and 3 walk the blocks backwards (xcd_block_dir): consecutive colour
+// launches then start where the previous one ended.
 template <bool HEATBATH>
 __global__ void LT_LINK_KERNEL sweep_cb4_kernel(double2* __restrict__ U, Geom g, int mu, int colour,
                                                  double prefactor, uint64_t seed, uint64_t counter)
 {
     int c[4], s;
-    if (!cb4_site(g, mu, colour, xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x, c, s)) return;
+    if (!cb4_site(g, mu, colour, xcd_block_dir(blockIdx.x, gridDim.x, colour & 1) * BS + threadIdx.x, c, s)) return;
     update_link<HEATBATH>(U, g, s, c, mu, parity_of(c), prefactor, seed, counter);
 }
 
@@ -38,7 +40,7 @@ __global__ void LT_LINK_KERNEL hb_cb4_graph_kernel(double2* __restrict__ U, Geom
                                                    uint64_t seed, const uint64_t* __restrict__ state, int k)
 {
     int c[4], s;
-    if (!cb4_site(g, mu, colour, xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x, c, s)) return;
+    if (!cb4_site(g, mu, colour, xcd_block_dir(blockIdx.x, gridDim.x, colour & 1) * BS + threadIdx.x, c, s)) return;
     update_link<true>(U, g, s, c, mu, parity_of(c), prefactor, seed, state[0] + (uint64_t)k);
 }
 
@@ -49,7 +51,7 @@ __global__ void __launch_bounds__(BS) sweep_action_kernel(double2* __restrict__ 
                                                           double c_plaq, double c_rect)
 {
     int c[4], s;
-    if (!cb4_site(g, mu, colour, xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x, c, s)) return;
+    if (!cb4_site(g, mu, colour, xcd_block_dir(blockIdx.x, gridDim.x, colour & 1) * BS + threadIdx.x, c, s)) return;
     update_link_action<HEATBATH>(U, g, s, c, mu, parity_of(c), prefactor, seed, counter, c_plaq, c_rect);
 }
 

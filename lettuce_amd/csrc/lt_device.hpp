@@ -45,6 +45,19 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
 #endif
     return b;
 }
+// The same with each XCD's range walked backwards when rev is set.  Consecutive
+// launches over the same links alternate direction, so each starts on the links
+// the previous one touched last (still in the L2 and the memory-side cache).
+__device__ __forceinline__ int xcd_block_dir(int b, int nb, int rev)
+{
+#if LT_XCD_REMAP
+    if ((nb & 7) == 0) {
+        const int per = nb >> 3, loc = b >> 3;
+        return (b & 7) * per + (rev ? per - 1 - loc : loc);
+    }
+#endif
+    return rev ? nb - 1 - b : b;
+}
 
 // ---------------------------------------------------------------------------
 // complex double
@@ -1024,6 +1037,10 @@ __device__ __forceinline__ bool map_site_b(const Geom& g, const LinkMap& m, int 
 __device__ __forceinline__ bool map_site(const Geom& g, const LinkMap& m, int par, int c[4], int& s)
 {
     return map_site_b(g, m, xcd_block(blockIdx.x, gridDim.x), par, c, s);
+}
+__device__ __forceinline__ bool map_site_dir(const Geom& g, const LinkMap& m, int par, int rev, int c[4], int& s)
+{
+    return map_site_b(g, m, xcd_block_dir(blockIdx.x, gridDim.x, rev), par, c, s);
 }
 
 // Jacobi kernels over all links: the 8 (mu, parity) classes of one tile are
