@@ -215,16 +215,16 @@ def metad16(la, ctx, n_traj=3, n_step=20):
             "ms_per_force_eval": 1e3 * dt / n_eval, "kernels": kernels}
 
 
-def lattice8(la, ctx, n_update=200):
+def lattice8(la, ctx, n_update=200, graph=True):
     """8^4 at beta = 5.7: n_update HB + 4 OR updates from Haar(0), plaquette after each (the
     correctness configuration: launch-latency bound).  Timed twice from the same start:
     one call per sweep with a plaquette read-back per update, and lt_update_sweeps (the
     update captured once as a graph, plaquettes read back after the last update); both
-    give the same field bit for bit."""
+    give the same field bit for bit.  graph=False times the eager calls only."""
     shape = (8,) * 4
     out = {"lattice": list(shape), "beta": 5.7, "updates": n_update}
     fields = []
-    for mode in ("eager", "graph"):
+    for mode in ("eager", "graph") if graph else ("eager",):
         U = la.GaugeField(ctx, shape)
         U.SetToHaar(0)
         U.set_rng(0, 0)
@@ -250,8 +250,11 @@ def lattice8(la, ctx, n_update=200):
         out[f"mean_plaquette_last_100_{mode}"] = float(np.mean(plaq[-100:]) / (18.0 * 8 ** 4))
         fields.append(U.download())
         U.close()
-    out["ms_per_update"] = out["ms_per_update_graph"]
-    out["graph_equals_eager"] = bool(np.array_equal(fields[0], fields[1]))
+    if graph:
+        out["ms_per_update"] = out["ms_per_update_graph"]
+        out["graph_equals_eager"] = bool(np.array_equal(fields[0], fields[1]))
+    else:
+        out["ms_per_update"] = out["ms_per_update_eager"]
     return out
 
 
@@ -331,7 +334,7 @@ def main():
     ap.add_argument("--metad-only", action="store_true",
                     help="run only the 16^4 MetaD-HMC trajectories and print their line (kernel counter runs)")
     ap.add_argument("--eager", action="store_true",
-                    help="PT-MetaD steps as eager launches instead of graph replays (kernel tracing)")
+                    help="no hipGraph anywhere: PT-MetaD steps and 8^4 updates as eager launches (kernel tracing)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -523,7 +526,7 @@ def main():
     }
     U.close()
     print("# 8^4 correctness configuration", file=sys.stderr, flush=True)
-    result["lattice_8"] = lattice8(la, ctx)
+    result["lattice_8"] = lattice8(la, ctx, graph=not args.eager)
     print("# MetaD-HMC 16^4", file=sys.stderr, flush=True)
     result["metad_16"] = metad16(la, ctx)
     if not args.no_pt:
