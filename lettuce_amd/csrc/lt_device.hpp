@@ -1209,7 +1209,11 @@ __device__ __forceinline__ int shift2(const Geom& g, int s, const int c[4], int 
 
 // Rectangular<2>::StapleRect (rectangular_action.hpp:186-225): for nu = mu+1, mu+2,
 // mu+3 (mod 4) the six 1x2 staples of link (x, mu) -- orthogonal to it (two), extending
-// it forward (two), backward (two) -- in the reference's grouping and order.
+// it forward (two), backward (two) -- in the reference's grouping and order.  Every
+// rectangle is a product of five links, an SU(3) matrix, so only rows 0 and 1 are
+// carried along each chain and row 2 is added as conj(row0 x row1) (acc_su3_r01); the
+// forward pair is taken as a U^+ + b U^+ and the backward pair as U^+ a + U^+ b (the
+// reference's (a + b) U^+ and U^+ (a + b)), so every accumulated term is a unitary chain.
 __device__ __forceinline__ M3 staple_rect(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
                                           int par)
 {
@@ -1223,29 +1227,46 @@ __device__ __forceinline__ M3 staple_rect(const double2* __restrict__ U, const G
         const int nud = at(0, -1), nudd = at(0, -2), mup_nudd = at(1, -2), mup_nud = at(1, -1);
         // orthogonal: U_nu(x) U_nu(x+nu) U_mu(x+2nu) U_nu(x+mu+nu)^+ U_nu(x+mu)^+ and its mirror
         {
-            M3 a = mul(mul(ldu(U, g, s, nu, par), ldu(U, g, nup, nu, q)), ldu(U, g, nupp, mu, par));
-            a = mul_nd(mul_nd(a, ldu(U, g, mup_nup, nu, par)), ldu(U, g, mup, nu, q));
-            M3 b = mul_dn(ldu(U, g, nud, nu, q), dag(ldu(U, g, nudd, nu, par)));
-            b = mul(mul(mul(b, ldu(U, g, nudd, mu, par)), ldu(U, g, mup_nudd, nu, q)), ldu(U, g, mup_nud, nu, par));
-            acc(st, add(a, b));
+            M3 r = mul_r01(ldu2(U, g, s, nu, par), ldu(U, g, nup, nu, q));
+            r = mul_r01(r, ldu(U, g, nupp, mu, par));
+            r = mul_nd_r01(r, ldu(U, g, mup_nup, nu, par));
+            r = mul_nd_r01(r, ldu(U, g, mup, nu, q));
+            acc_su3_r01(st, r);
+            r = mul_dd_r01(ldu(U, g, nud, nu, q), ldu(U, g, nudd, nu, par));
+            r = mul_r01(r, ldu(U, g, nudd, mu, par));
+            r = mul_r01(r, ldu(U, g, mup_nudd, nu, q));
+            r = mul_r01(r, ldu(U, g, mup_nud, nu, par));
+            acc_su3_r01(st, r);
         }
         // forward: [U_nu(x) U_mu(x+nu) U_mu(x+mu+nu) U_nu(x+2mu)^+ + mirror] U_mu(x+mu)^+
         {
             const int mupp = at(2, 0), mupp_nud = at(2, -1);
-            M3 a = mul_nd(mul(mul(ldu(U, g, s, nu, par), ldu(U, g, nup, mu, q)), ldu(U, g, mup_nup, mu, par)),
-                          ldu(U, g, mupp, nu, par));
-            M3 b = mul(mul(mul_dn(ldu(U, g, nud, nu, q), ldu(U, g, nud, mu, q)), ldu(U, g, mup_nud, mu, par)),
-                       ldu(U, g, mupp_nud, nu, q));
-            acc(st, mul_nd(add(a, b), ldu(U, g, mup, mu, q)));
+            const M3 ump = ldu(U, g, mup, mu, q);
+            M3 r = mul_r01(ldu2(U, g, s, nu, par), ldu(U, g, nup, mu, q));
+            r = mul_r01(r, ldu(U, g, mup_nup, mu, par));
+            r = mul_nd_r01(r, ldu(U, g, mupp, nu, par));
+            r = mul_nd_r01(r, ump);
+            acc_su3_r01(st, r);
+            r = mul_dn_r01(ldu(U, g, nud, nu, q), ldu(U, g, nud, mu, q));
+            r = mul_r01(r, ldu(U, g, mup_nud, mu, par));
+            r = mul_r01(r, ldu(U, g, mupp_nud, nu, q));
+            r = mul_nd_r01(r, ump);
+            acc_su3_r01(st, r);
         }
         // backward: U_mu(x-mu)^+ [U_nu(x-mu) U_mu(x-mu+nu) U_mu(x+nu) U_nu(x+mu)^+ + mirror]
         {
             const int mud = at(-1, 0), mud_nup = at(-1, 1), mud_nud = at(-1, -1);
-            M3 a = mul_nd(mul(mul(ldu(U, g, mud, nu, q), ldu(U, g, mud_nup, mu, par)), ldu(U, g, nup, mu, q)),
-                          ldu(U, g, mup, nu, q));
-            M3 b = mul(mul(mul_dn(ldu(U, g, mud_nud, nu, par), ldu(U, g, mud_nud, mu, par)), ldu(U, g, nud, mu, q)),
-                       ldu(U, g, mup_nud, nu, par));
-            acc(st, mul_dn(ldu(U, g, mud, mu, q), add(a, b)));
+            const M3 umd = ldu(U, g, mud, mu, q);
+            M3 r = mul_dn_r01(umd, ldu(U, g, mud, nu, q));
+            r = mul_r01(r, ldu(U, g, mud_nup, mu, par));
+            r = mul_r01(r, ldu(U, g, nup, mu, q));
+            r = mul_nd_r01(r, ldu(U, g, mup, nu, q));
+            acc_su3_r01(st, r);
+            r = mul_dd_r01(umd, ldu(U, g, mud_nud, nu, par));
+            r = mul_r01(r, ldu(U, g, mud_nud, mu, par));
+            r = mul_r01(r, ldu(U, g, nud, mu, q));
+            r = mul_r01(r, ldu(U, g, mup_nud, nu, par));
+            acc_su3_r01(st, r);
         }
     }
     return st;
