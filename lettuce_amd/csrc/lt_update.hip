@@ -96,6 +96,17 @@ __global__ void LT_LINK_KERNEL staple_field_kernel(const double2* __restrict__ U
     st_link(out, g, s, mu, par, staple(U, g, s, c, mu));
 }
 
+// The colour launches walk all y tiles of an x step in one chunk (LinkMap::yc = N[2]):
+// with the alternating walk (rev) that is 2 % faster per HB + 4 OR update at 32^4 than
+// the two-tile chunks the Jacobi kernels keep (the RK3 flow stage is 1.5 % slower with one
+// chunk).  The tile order never changes a result.
+LinkMap sweep_map(const LinkMap& m)
+{
+    LinkMap r = m;
+    r.yc = r.N[2];
+    return r;
+}
+
 // n sweeps; the four-colour ones (split) expect the field in the split order
 template <bool HEATBATH>
 void sweep_launches(lt_field u, int n, int schedule, double prefactor, const char* name)
@@ -108,7 +119,7 @@ void sweep_launches(lt_field u, int n, int schedule, double prefactor, const cha
             if (schedule == LT_CB4) {
                 for (int col = 0; col < 4; ++col) k_cb4_colour(u, HEATBATH, mu, col, prefactor, counter, name);
             } else {
-                const LinkMap& m = u->map;
+                const LinkMap m = sweep_map(u->map);
                 for (int par = 0; par < 2; ++par)
                     launch(ctx, name, sweep_cb2_kernel<HEATBATH>, dim3(map_blocks(m)), dim3(m.nthreads), 0, u->d, g, m, mu,
                            par, prefactor, u->seed, counter, par);
@@ -220,7 +231,7 @@ void enqueue_update(lt_field u, UpdateGraph& G, double prefactor)
 {
     lt_ctx ctx = u->ctx;
     const Geom& g = u->g;
-    const LinkMap& m = u->map;
+    const LinkMap m = sweep_map(u->map);
     for (int k = 0; k < G.n_hb; ++k)
         for (int mu = 0; mu < 4; ++mu) {
             if (G.schedule == LT_CB4) {
