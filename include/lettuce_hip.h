@@ -108,8 +108,10 @@ int lt_field_bmw_checksum(lt_field f, int full_rows, unsigned long long *checksu
 int lt_field_set_rng(lt_field f, uint64_t seed, uint64_t counter);
 int lt_field_get_rng(lt_field f, uint64_t *seed, uint64_t *counter);
 /* launch shape of the link kernels: thread blocks cover 4D tiles of extent
- * (tt, tx, ty, tz) in (t, x, y, z/2), which must divide the lattice; results do
- * not depend on it (no reference counterpart: the reference's OpenMP schedule) */
+ * (tt, tx, ty, tz) in (t, x, y, z/2), which must divide the lattice, or, with
+ * tt = tx = ty = 0, tz consecutive checkerboard sites (tz = 64, 128, 192 or 256);
+ * results do not depend on it (no reference counterpart: the reference's OpenMP
+ * schedule) */
 int lt_field_set_tile(lt_field f, int tt, int tx, int ty, int tz);
 int lt_field_get_tile(lt_field f, int tile[4]);
 

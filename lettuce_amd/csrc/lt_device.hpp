@@ -994,18 +994,32 @@ __device__ __forceinline__ int parity_of(const int c[4]) { return (c[0] + c[1] +
 // contiguous block range is one t-slab and the t-neighbours of a tile run a few
 // blocks apart, inside the set of blocks resident in that XCD's L2.  The block
 // index is first spread over XCDs.
+//
+// Linear maps (lin = 1) cover instead nthreads consecutive checkerboard sites of the
+// parity per block (T = (1, 1, 1, nthreads), N = (blocks, 1, 1, 1)).  They serve extents
+// whose z/2 has few factors of two (12^4, 20^4, 24^4: a dividing power-of-two box holds 2
+// or 4 sites of a row, so a wavefront reads 32- or 64-byte pieces of 32 rows), where
+// consecutive sites read whole rows and the t-slab of each XCD stays contiguous.
 struct LinkMap {
     int T[4];     // tile extent in t, x, y, zc
     int N[4];     // number of tiles in t, x, y, zc
     int nthreads; // T[0]*T[1]*T[2]*T[3] == blockDim.x
     int tc;       // t tiles per chunk (divides N[0])
     int yc;       // y tiles per chunk (divides N[2])
+    int lin;      // linear map: block b covers checkerboard sites [b * nthreads, (b + 1) * nthreads)
 };
 
 // tile order, fastest first: t within its chunk, z, y within its chunk, x, y chunk, t chunk
+__device__ __forceinline__ int site_of_cb(const Geom& g, int cb, int par, int c[4]);
 __device__ __forceinline__ bool map_site_b(const Geom& g, const LinkMap& m, int b, int par, int c[4], int& s)
 {
     int tid = threadIdx.x;
+    if (m.lin) {
+        const int cb = b * m.nthreads + tid;
+        if (tid >= m.nthreads || cb >= g.Vh) return false;
+        s = site_of_cb(g, cb, par, c);
+        return true;
+    }
     const int bt_lo = b % m.tc;
     b /= m.tc;
     int bz = b % m.N[3];

@@ -51,30 +51,27 @@ __device__ __forceinline__ SU2 overrelax(SU2 A)
     return SU2{conj(VV.e11), -VV.e12};
 }
 
-// Kennedy-Pendleton / Fabricius-Haan SU(2) sampler in the reference's draw order:
-// attempt a uses Philox blocks 2a (r1, r2) and 2a+1 (r3, r0); the direction uses
-// block 0xFFFFF.  After max_iter+1 rejected attempts the identity is returned.
-__device__ __forceinline__ SU2 heatbath_su2(SU2 A, double prefactor, uint64_t seed, uint32_t site, uint32_t sub,
-                                            uint32_t mu, uint64_t counter)
+// One Kennedy-Pendleton / Fabricius-Haan attempt k of the SU(2) sampler in the
+// reference's draw order: Philox blocks 2k (r1, r2) and 2k+1 (r3, r0) of the site;
+// coef = -0.25 prefactor / sqrt(det A).  Returns whether the attempt is accepted.
+__device__ __forceinline__ bool kp_attempt(uint64_t seed, uint32_t site, uint32_t sub, uint32_t mu, uint64_t counter,
+                                           int k, double coef, double& lambda_sq)
 {
-    constexpr int max_iter = 10;
-    const double an = 1.0 / sqrt(su2_det(A));
-    SU2 V{an * A.e11, an * A.e12};
-    double lambda_sq = 0.0;
-    int it = 0;
-    for (;;) {
-        double ua0, ua1, ub0, ub1;
-        draw2(seed, site, RNG_HEATBATH, (uint32_t)(2 * it), sub, mu, counter, ua0, ua1);
-        draw2(seed, site, RNG_HEATBATH, (uint32_t)(2 * it + 1), sub, mu, counter, ub0, ub1);
-        double x1 = log(1.0 - ua0);
-        double x2 = cos(2.0 * 3.14159265358979323846 * (1.0 - ua1));
-        double x3 = log(1.0 - ub0);
-        lambda_sq = (-0.25 * prefactor * an) * (x1 + x2 * x2 * x3);
-        double r0 = ub1;
-        if (it > max_iter) return SU2{cd{1.0, 0.0}, cd{0.0, 0.0}};
-        ++it;
-        if (!(r0 * r0 + lambda_sq >= 1.0)) break;
-    }
+    double ua0, ua1, ub0, ub1;
+    draw2(seed, site, RNG_HEATBATH, (uint32_t)(2 * k), sub, mu, counter, ua0, ua1);
+    draw2(seed, site, RNG_HEATBATH, (uint32_t)(2 * k + 1), sub, mu, counter, ub0, ub1);
+    const double x1 = log(1.0 - ua0);
+    const double x2 = cos(2.0 * 3.14159265358979323846 * (1.0 - ua1));
+    const double x3 = log(1.0 - ub0);
+    lambda_sq = coef * (x1 + x2 * x2 * x3);
+    const double r0 = ub1;
+    return !(r0 * r0 + lambda_sq >= 1.0);
+}
+
+// The accepted SU(2) draw X (direction from Philox block 0xFFFFF) times V^+, V = A / sqrt(det A)
+__device__ __forceinline__ SU2 kp_finish(SU2 V, double lambda_sq, uint64_t seed, uint32_t site, uint32_t sub,
+                                         uint32_t mu, uint64_t counter)
+{
     double x0 = 1.0 - 2.0 * lambda_sq;
     double abs_x = sqrt(1.0 - x0 * x0);
     double phi, ct;
@@ -86,6 +83,24 @@ __device__ __forceinline__ SU2 heatbath_su2(SU2 A, double prefactor, uint64_t se
     SU2 X{cd{x0, vec_norm * cp}, cd{vec_norm * sp, abs_x * cos_theta}};
     SU2 Vd{conj(V.e11), -V.e12};
     return su2_mul(X, Vd);
+}
+
+// Kennedy-Pendleton / Fabricius-Haan SU(2) sampler in the reference's draw order: the
+// first accepted attempt k = 0, 1, ..., max_iter; after max_iter+1 rejected attempts
+// the identity is returned.
+constexpr int KP_MAX_ITER = 10;
+__device__ __forceinline__ SU2 heatbath_su2(SU2 A, double prefactor, uint64_t seed, uint32_t site, uint32_t sub,
+                                            uint32_t mu, uint64_t counter)
+{
+    const double an = 1.0 / sqrt(su2_det(A));
+    SU2 V{an * A.e11, an * A.e12};
+    const double coef = -0.25 * prefactor * an;
+    double lambda_sq = 0.0;
+    for (int k = 0;; ++k) {
+        if (k > KP_MAX_ITER) return SU2{cd{1.0, 0.0}, cd{0.0, 0.0}};
+        if (kp_attempt(seed, site, sub, mu, counter, k, coef, lambda_sq)) break;
+    }
+    return kp_finish(V, lambda_sq, seed, site, sub, mu, counter);
 }
 
 // Site i in [0, V/4) of colour k of direction mu under Checkerboard4, where the colour

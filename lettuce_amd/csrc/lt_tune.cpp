@@ -39,6 +39,11 @@ std::vector<LinkMap> tile_candidates(const Geom& g, int threads)
             }
         }
     }
+    // linear maps of consecutive checkerboard sites
+    for (int n : {64, 128, 256}) {
+        LinkMap m{};
+        if ((threads <= 0 || n == threads) && make_linkmap(g, 0, 0, 0, n, m)) out.push_back(m);
+    }
     if (out.empty()) out.push_back(default_linkmap(g));
     return out;
 }
@@ -56,7 +61,8 @@ std::string tune_key(int device, const Geom& g, const char* family)
     return s.str();
 }
 
-// Cache file: one JSON object, "key": [tt, tx, ty, tz, ms] per entry.
+// Cache file: one JSON object, "key": [tt, tx, ty, tz, ms] per entry (tt = tx = ty = 0: a
+// linear map of tz consecutive checkerboard sites per block).
 static std::map<std::string, std::array<double, 5>> read_cache(const char* path)
 {
     std::map<std::string, std::array<double, 5>> m;
@@ -115,7 +121,7 @@ void store_tile(const char* path, const std::string& key, const LinkMap& m, doub
 {
     if (!path || !*path) return;
     auto c = read_cache(path);
-    c[key] = {(double)m.T[0], (double)m.T[1], (double)m.T[2], (double)m.T[3], ms};
+    c[key] = {m.lin ? 0.0 : (double)m.T[0], m.lin ? 0.0 : (double)m.T[1], m.lin ? 0.0 : (double)m.T[2], (double)m.T[3], ms};
     write_cache(path, c);
 }
 
@@ -215,7 +221,7 @@ extern "C" int lt_autotune_field(lt_field f, int family, const char* cache_path,
         }
         f->map = m;
         if (tile_out)
-            for (int i = 0; i < 4; ++i) tile_out[i] = m.T[i];
+            for (int i = 0; i < 4; ++i) tile_out[i] = m.lin && i < 3 ? 0 : m.T[i];
         if (best_ms) *best_ms = ms;
         return LT_OK;
     });

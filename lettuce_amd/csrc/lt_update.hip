@@ -47,6 +47,114 @@ hb_cb2_graph_kernel(double2* __restrict__ U, Geom g, LinkMap m, int mu, int par,
     update_link<true>(U, g, s, c, mu, par, prefactor, seed, state[0] + (uint64_t)k);
 }
 
+// ---- small lattices ---------------------------------------------------------------
+// A colour launch of a few thousand links leaves most of the GPU idle, and its duration
+// is one wavefront's serial work: about 2400 VALU instructions for an overrelaxation
+// link, twice that with the heatbath sampler (8^4: 7.8 and 17.8 us per launch).  Up to
+// SMALL_VH links per colour launch, four lanes share a link: lane j < 3 builds the upper
+// and the lower staple term of the j-th direction nu != mu, a two-step sum inside the
+// quad (DPP quad permutations; every lane ends with the same ((t0 + t1) + (t2 + 0)),
+// since a + b = b + a) gives all four lanes the staple, all four apply the SU(2) steps
+// -- the heatbath's attempts base + j run side by side and the first accepted one is
+// taken, exactly the attempt the sequential loop would take -- and lane 0 stores.
+#ifndef LT_SMALL_VH
+#define LT_SMALL_VH 16384
+#endif
+constexpr int SMALL_VH = LT_SMALL_VH;
+constexpr int SBS = 256;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// a <- a + a of lane (lane ^ 1) (CTRL 0xB1) or (lane ^ 2) (CTRL 0x4E)
+template <int CTRL>
+__device__ __forceinline__ void quad_add(M3& a)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        a.m[k].re += dpp_f64<CTRL>(a.m[k].re);
+        a.m[k].im += dpp_f64<CTRL>(a.m[k].im);
+    }
+}
+
+// upper + lower staple term of direction nu for link (s, mu)
+__device__ __forceinline__ M3 staple_pair(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
+                                          int par, int nu)
+{
+    const int q = par ^ 1;
+    const int cnu = pick4(c, nu), Ln = pick4(g.L, nu), stn = pick4(g.st, nu);
+    const int s_nu = (cnu + 1 < Ln) ? s + stn : s - (Ln - 1) * stn;
+    const int s_nd = (cnu > 0) ? s - stn : s + (Ln - 1) * stn;
+    const int cmu = pick4(c, mu), Lm = pick4(g.L, mu), stm = pick4(g.st, mu);
+    const int dmu = (cmu + 1 < Lm) ? stm : -(Lm - 1) * stm;   // site step +mu
+    const int s_mu = s + dmu, s_nd_mu = s_nd + dmu;
+    M3 t = mul_nd_r01(mul_r01(ldu2(U, g, s, nu, par), ldu(U, g, s_nu, mu, q)), ldu(U, g, s_mu, nu, q));
+    third_row(t);
+    M3 ef = mul_r01(ldu2(U, g, s_nd, mu, q), ldu(U, g, s_nd_mu, nu, par));
+    third_row(ef);
+    mul_dn_acc(t, ldu(U, g, s_nd, nu, q), ef);
+    return t;
+}
+
+// One colour launch (mu, parity) of a small lattice, four lanes per link; the heatbath
+// draws with counter state[0] + k when state is given (captured updates), else counter.
+template <bool HEATBATH>
+__global__ void __launch_bounds__(SBS) sweep_small_kernel(double2* __restrict__ U, Geom g, int mu, int par,
+                                                          double prefactor, uint64_t seed, uint64_t counter,
+                                                          const uint64_t* __restrict__ state, int k)
+{
+    const int tid = blockIdx.x * SBS + threadIdx.x;
+    const int l = tid >> 2, j = tid & 3;
+    if (l >= g.Vh) return;   // whole quads (4 Vh threads)
+    if (HEATBATH && state) counter = state[0] + (uint64_t)k;
+    int c[4];
+    const int s = site_of_cb(g, l, par, c);
+    M3 st = j < 3 ? staple_pair(U, g, s, c, mu, par, j + (j >= mu)) : m3_zero();
+    quad_add<0xB1>(st);
+    quad_add<0x4E>(st);
+    M3 L = ldu(U, g, s, mu, par);
+    const int lane = __lane_id();
+#pragma unroll
+    for (int sub = 0; sub < 3; ++sub) {
+        const int i = sub == 2 ? 1 : 0;
+        const int jj = sub == 0 ? 1 : 2;
+        const SU2 A = extract(L, st, i, jj);
+        SU2 X;
+        if (HEATBATH) {
+            const double an = 1.0 / sqrt(su2_det(A));
+            const SU2 V{an * A.e11, an * A.e12};
+            const double coef = -0.25 * prefactor * an;
+            double lambda_sq = 0.0;
+            bool found = false;
+            for (int base = 0; base <= KP_MAX_ITER; base += 4) {
+                double lsq = 0.0;
+                const int kk = base + j;
+                const bool acc = kk <= KP_MAX_ITER &&
+                                 kp_attempt(seed, (uint32_t)s, (uint32_t)sub, (uint32_t)mu, counter, kk, coef, lsq);
+                const int qbits = (int)((__ballot(acc) >> (lane & ~3)) & 0xFull);
+                const double first = __shfl(lsq, qbits ? (lane & ~3) + __ffs(qbits) - 1 : lane);
+                if (qbits) {
+                    lambda_sq = first;
+                    found = true;
+                    break;
+                }
+            }
+            X = found ? kp_finish(V, lambda_sq, seed, (uint32_t)s, (uint32_t)sub, (uint32_t)mu, counter)
+                      : SU2{cd{1.0, 0.0}, cd{0.0, 0.0}};
+        } else {
+            X = overrelax(A);
+        }
+        embed_mul(L, X, i, jj);
+    }
+    gram_schmidt(L);
+    if (j == 0) stu(U, g, s, mu, par, L);
+}
+
 // PlaquetteSum of the captured update: per-block sums (the block tree of the HMC
 // reductions), then one ordered sum written to plaq[state[1]]
 constexpr int RBS = 256;
@@ -118,6 +226,10 @@ void sweep_launches(lt_field u, int n, int schedule, double prefactor, const cha
         for (int mu = 0; mu < 4; ++mu) {
             if (schedule == LT_CB4) {
                 for (int col = 0; col < 4; ++col) k_cb4_colour(u, HEATBATH, mu, col, prefactor, counter, name);
+            } else if (g.Vh <= SMALL_VH) {
+                for (int par = 0; par < 2; ++par)
+                    launch(ctx, name, sweep_small_kernel<HEATBATH>, dim3(nblocks(4LL * g.Vh, SBS)), dim3(SBS), 0, u->d, g,
+                           mu, par, prefactor, u->seed, counter, (const uint64_t*)nullptr, 0);
             } else {
                 const LinkMap m = sweep_map(u->map);
                 for (int par = 0; par < 2; ++par)
@@ -223,7 +335,7 @@ bool same_map(const LinkMap& a, const LinkMap& b)
 {
     for (int i = 0; i < 4; ++i)
         if (a.T[i] != b.T[i] || a.N[i] != b.N[i]) return false;
-    return a.nthreads == b.nthreads && a.tc == b.tc && a.yc == b.yc;
+    return a.nthreads == b.nthreads && a.tc == b.tc && a.yc == b.yc && a.lin == b.lin;
 }
 
 // the launches of one update, in the order of the eager sweeps
@@ -236,6 +348,10 @@ void enqueue_update(lt_field u, UpdateGraph& G, double prefactor)
         for (int mu = 0; mu < 4; ++mu) {
             if (G.schedule == LT_CB4) {
                 for (int col = 0; col < 4; ++col) k_cb4_colour_graph_hb(u, mu, col, prefactor, G.state, k);
+            } else if (g.Vh <= SMALL_VH) {
+                for (int par = 0; par < 2; ++par)
+                    launch(ctx, "hb_cb2", sweep_small_kernel<true>, dim3(nblocks(4LL * g.Vh, SBS)), dim3(SBS), 0, u->d, g,
+                           mu, par, prefactor, u->seed, (uint64_t)0, (const uint64_t*)G.state, k);
             } else {
                 for (int par = 0; par < 2; ++par)
                     launch(ctx, "hb_cb2", hb_cb2_graph_kernel, dim3(map_blocks(m)), dim3(m.nthreads), 0, u->d, g, m,
