@@ -101,6 +101,30 @@ LinkMap default_linkmap(const Geom& g)
             want /= 2;
         }
     }
+    // rows cut into pieces of 8 or more (z/2 = 24 at 48^4): a box of whole rows whose
+    // thread count is a multiple of 64 (48^4: 1 x 1 x 8 x 24, HB + OR sweeps -4 %, RK3
+    // stage -5 % against 1 x 1 x 16 x 8), most y rows first
+    if (want == 1 && T[3] >= 8 && T[3] < ext[3] && ext[3] <= 32) {
+        int bestn = 0, B[4] = {0, 0, 0, 0};
+        for (int ty = ext[2]; ty >= 1; --ty) {
+            if (ext[2] % ty) continue;
+            for (int tx = 1; tx <= ext[1]; ++tx) {
+                if (ext[1] % tx) continue;
+                for (int tt = 1; tt <= ext[0]; ++tt) {
+                    if (ext[0] % tt) continue;
+                    const int n = tt * tx * ty * ext[3];
+                    if (n % 64 == 0 && n <= 256 && n > bestn) {
+                        bestn = n;
+                        B[0] = tt;
+                        B[1] = tx;
+                        B[2] = ty;
+                        B[3] = ext[3];
+                    }
+                }
+            }
+        }
+        if (bestn >= 128 && make_linkmap(g, B[0], B[1], B[2], B[3], m)) return m;
+    }
     if (want == 1 && (T[3] >= 8 || T[3] == ext[3]) && make_linkmap(g, T[0], T[1], T[2], T[3], m)) return m;
     if (make_linkmap(g, 0, 0, 0, 128, m)) return m;
     // extents without enough factors of two: the largest dividing tile of at most
