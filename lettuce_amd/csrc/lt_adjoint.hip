@@ -24,7 +24,7 @@ constexpr int CBS = 256;
 __global__ void LT_LINK_KERNEL stout_tape_kernel(const double2* __restrict__ Uin, double2* __restrict__ Uout,
                                                  double2* __restrict__ T, Geom g, LinkMap m, double rho)
 {
-    body_stout_tape(Uin, Uout, T, g, m, xcd_block(blockIdx.x, gridDim.x), rho);
+    body_stout_tape(Uin, Uout, T, g, m, link_block(m, blockIdx.x, gridDim.x), rho);
 }
 
 // one thread per (site, plane): the clover term of that plane
@@ -52,27 +52,27 @@ template <int HALF, int K>
 __global__ void LT_LINK_KERNEL clover_deriv_kernel(const double2* __restrict__ U, const double2* __restrict__ C,
                                                    double2* __restrict__ F, Geom g, LinkMap m, double scale_factor)
 {
-    body_clover_deriv<HALF, K>(U, C, F, g, m, xcd_block(blockIdx.x, gridDim.x), scale_factor);
+    body_clover_deriv<HALF, K>(U, C, F, g, m, link_block(m, blockIdx.x, gridDim.x), scale_factor);
 }
 
 __global__ void LT_LINK_KERNEL lambda_kernel(const double2* __restrict__ U, const double2* __restrict__ Uprev,
                                              double2* __restrict__ Sig, const double2* __restrict__ T,
                                              double2* __restrict__ Lam, Geom g, LinkMap m)
 {
-    body_lambda(U, Uprev, Sig, T, Lam, g, m, xcd_block(blockIdx.x, gridDim.x));
+    body_lambda(U, Uprev, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x));
 }
 
 __global__ void LT_LINK_KERNEL sigma_fs_kernel(const double2* __restrict__ U, const double2* __restrict__ Lam,
                                                double2* __restrict__ Fs, Geom g, LinkMap m)
 {
-    body_sigma_fs(U, Lam, Fs, g, m, xcd_block(blockIdx.x, gridDim.x));
+    body_sigma_fs(U, Lam, Fs, g, m, link_block(m, blockIdx.x, gridDim.x));
 }
 
 __global__ void LT_LINK_KERNEL sigma_kernel(const double2* __restrict__ U, double2* __restrict__ Sig,
                                             const double2* __restrict__ T, const double2* __restrict__ Lam,
                                             const double2* __restrict__ Fs, Geom g, LinkMap m, double rho)
 {
-    body_sigma(U, Sig, T, Lam, Fs, g, m, xcd_block(blockIdx.x, gridDim.x), rho);
+    body_sigma(U, Sig, T, Lam, Fs, g, m, link_block(m, blockIdx.x, gridDim.x), rho);
 }
 
 }  // namespace

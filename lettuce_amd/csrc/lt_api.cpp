@@ -42,7 +42,8 @@ bool make_linkmap(const Geom& g, int tt, int tx, int ty, int tz, LinkMap& m)
         m.T[3] = tz;
         m.N[0] = (g.Vh + tz - 1) / tz;
         m.N[1] = m.N[2] = m.N[3] = 1;
-        m.tc = m.yc = 1;
+        m.tc = m.yc = m.ycs = 1;
+        m.xr = LT_XCD_REMAP;
         return true;
     }
     const int ext[4] = {g.L[0], g.L[1], g.L[2], g.L[3] / 2};
@@ -70,6 +71,9 @@ bool make_linkmap(const Geom& g, int tt, int tx, int ty, int tz, LinkMap& m)
 #if LT_YCHUNK > 0
     if (m.N[2] % LT_YCHUNK == 0) m.yc = LT_YCHUNK;
 #endif
+    // the colour sweeps walk all y tiles of an x step in one chunk
+    m.ycs = m.N[2];
+    m.xr = LT_XCD_REMAP;
     return true;
 }
 

@@ -43,7 +43,7 @@ __device__ __forceinline__ int rep_of(const int* __restrict__ pre, int n, int b)
     const Geom g = bg.reps[r].g;                                                \
     const LinkMap m = bg.reps[r].m;                                             \
     const long long off = bg.reps[r].off;                                       \
-    const int lb = xcd_block(blockIdx.x - bg.lb[r], bg.lb[r + 1] - bg.lb[r]);
+    const int lb = link_block(m, blockIdx.x - bg.lb[r], bg.lb[r + 1] - bg.lb[r]);
 
 __global__ void LT_LINK_KERNEL bk_momentum(double2* __restrict__ P, BatchGrid bg, const uint64_t* __restrict__ counter)
 {

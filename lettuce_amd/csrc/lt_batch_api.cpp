@@ -487,7 +487,11 @@ int lt_batch_autotune(lt_batch b, const char* cache_path)
                 for (int q : same) b->reps[q].m = m;
                 upload(b->d_reps, b->reps.data(), sizeof(lt::BatchRep) * b->n);
             };
-            if (!lt::cached_tile(cache_path, key, g, best, ms)) {
+            // the replica's block range was sized for the map in use: a new map must need the
+            // same number of blocks (fewer would leave links out)
+            const lt::LinkMap cur = b->reps[same[0]].m;
+            const unsigned nb = lt::map_blocks(cur);
+            if (!lt::cached_tile(cache_path, key, g, best, ms) || lt::map_blocks(best) != nb) {
                 const lt::BatchGrid bg = b->grid();
                 auto run = [&](const lt::LinkMap& m) {
                     apply(m);
@@ -497,7 +501,12 @@ int lt_batch_autotune(lt_batch b, const char* cache_path)
                     lt::kb_stout_recursion(b->ctx, bg, b->total_lb, b->U, b->U, b->F, tape, b->lambda, b->fsum,
                                            b->rho);
                 };
-                best = lt::tune(b->ctx, lt::tile_candidates(g, b->nthreads), run, ms, 2);
+                std::vector<lt::LinkMap> cands;
+                for (const lt::LinkMap& c : lt::tile_candidates(g, cur.nthreads))
+                    if (lt::map_blocks(c) == nb) cands.push_back(c);
+                if (cands.empty()) cands.push_back(cur);
+                best = lt::tune(b->ctx, cands, cur, run, ms, 2);
+                best = lt::tune(b->ctx, lt::order_candidates(best, false), best, run, ms, 2);
                 lt::store_tile(cache_path, key, best, ms);
             }
             apply(best);

@@ -1005,9 +1005,19 @@ struct LinkMap {
     int N[4];     // number of tiles in t, x, y, zc
     int nthreads; // T[0]*T[1]*T[2]*T[3] == blockDim.x
     int tc;       // t tiles per chunk (divides N[0])
-    int yc;       // y tiles per chunk (divides N[2])
+    int yc;       // y tiles per chunk (divides N[2]) of the Jacobi kernels (flow, stout, MetaD)
     int lin;      // linear map: block b covers checkerboard sites [b * nthreads, (b + 1) * nthreads)
+    int ycs;      // y tiles per chunk of the colour sweeps
+    int xr;       // spread the block index over the XCDs (xcd_block)
 };
+
+// Block index of a link kernel in its map's order: spread over the XCDs when m.xr is
+// set, walked backwards when rev is set.
+__device__ __forceinline__ int link_block(const LinkMap& m, int b, int nb, int rev = 0)
+{
+    if (m.xr) return xcd_block_dir(b, nb, rev);
+    return rev ? nb - 1 - b : b;
+}
 
 // tile order, fastest first: t within its chunk, z, y within its chunk, x, y chunk, t chunk
 __device__ __forceinline__ int site_of_cb(const Geom& g, int cb, int par, int c[4]);
@@ -1050,11 +1060,11 @@ __device__ __forceinline__ bool map_site_b(const Geom& g, const LinkMap& m, int 
 
 __device__ __forceinline__ bool map_site(const Geom& g, const LinkMap& m, int par, int c[4], int& s)
 {
-    return map_site_b(g, m, xcd_block(blockIdx.x, gridDim.x), par, c, s);
+    return map_site_b(g, m, link_block(m, blockIdx.x, gridDim.x), par, c, s);
 }
 __device__ __forceinline__ bool map_site_dir(const Geom& g, const LinkMap& m, int par, int rev, int c[4], int& s)
 {
-    return map_site_b(g, m, xcd_block_dir(blockIdx.x, gridDim.x, rev), par, c, s);
+    return map_site_b(g, m, link_block(m, blockIdx.x, gridDim.x, rev), par, c, s);
 }
 
 // Jacobi kernels over all links: the 8 (mu, parity) classes of one tile are
@@ -1069,7 +1079,7 @@ __device__ __forceinline__ bool map_link_b(const Geom& g, const LinkMap& m, int 
 }
 __device__ __forceinline__ bool map_link_all(const Geom& g, const LinkMap& m, int& mu, int& par, int c[4], int& s)
 {
-    return map_link_b(g, m, xcd_block(blockIdx.x, gridDim.x), mu, par, c, s);
+    return map_link_b(g, m, link_block(m, blockIdx.x, gridDim.x), mu, par, c, s);
 }
 
 // Site of the cb-th entry of a parity: s = 2*cb or 2*cb+1 with the right parity.

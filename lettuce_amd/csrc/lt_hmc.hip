@@ -16,19 +16,19 @@ constexpr int RBS = 256;   // reduction block size
 __global__ void LT_LINK_KERNEL momentum_kernel(double2* __restrict__ P, Geom g, LinkMap m, uint64_t seed,
                                                uint64_t counter)
 {
-    body_momentum(P, g, m, xcd_block(blockIdx.x, gridDim.x), seed, counter);
+    body_momentum(P, g, m, link_block(m, blockIdx.x, gridDim.x), seed, counter);
 }
 
 __global__ void LT_LINK_KERNEL force_update_kernel(const double2* __restrict__ U, double2* __restrict__ P,
                                                    const double2* __restrict__ F, Geom g, LinkMap m, double a, double b)
 {
-    body_force_update(U, P, F, g, m, xcd_block(blockIdx.x, gridDim.x), a, b);
+    body_force_update(U, P, F, g, m, link_block(m, blockIdx.x, gridDim.x), a, b);
 }
 
 __global__ void LT_LINK_KERNEL field_update_kernel(double2* __restrict__ U, const double2* __restrict__ P, Geom g,
                                                    LinkMap m, double eps)
 {
-    body_field_update(U, P, g, m, xcd_block(blockIdx.x, gridDim.x), eps);
+    body_field_update(U, P, g, m, link_block(m, blockIdx.x, gridDim.x), eps);
 }
 
 __device__ __forceinline__ void block_sum(double v, double* out)

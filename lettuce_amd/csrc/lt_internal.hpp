@@ -186,7 +186,7 @@ inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / 
 inline unsigned map_blocks(const LinkMap& m)
 {
     if (m.nthreads < 1 || m.nthreads > LT_LINK_BS || m.T[0] * m.T[1] * m.T[2] * m.T[3] != m.nthreads || m.tc < 1
-        || m.N[0] % m.tc || m.yc < 1 || m.N[2] % m.yc)
+        || m.N[0] % m.tc || m.yc < 1 || m.N[2] % m.yc || m.ycs < 1 || m.N[2] % m.ycs)
         throw LtError{LT_ESTATE, "invalid thread-block tile for the link kernels"};
     return (unsigned)(m.N[0] * m.N[1] * m.N[2] * m.N[3]);
 }
@@ -219,6 +219,14 @@ Geom make_geom(int nt, int nx, int ny, int nz);
 // tile map with extents (tt, tx, ty, tz) over (t, x, y, z/2); returns false if they do not divide
 bool make_linkmap(const Geom& g, int tt, int tx, int ty, int tz, LinkMap& m);
 LinkMap default_linkmap(const Geom& g);
+// same block shape and site order
+inline bool same_map(const LinkMap& a, const LinkMap& b)
+{
+    for (int i = 0; i < 4; ++i)
+        if (a.T[i] != b.T[i] || a.N[i] != b.N[i]) return false;
+    return a.nthreads == b.nthreads && a.tc == b.tc && a.yc == b.yc && a.lin == b.lin && a.ycs == b.ycs &&
+           a.xr == b.xr;
+}
 
 // kernel launchers (defined in the .hip translation units)
 void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n);

@@ -39,12 +39,59 @@ std::vector<LinkMap> tile_candidates(const Geom& g, int threads)
             }
         }
     }
+    // boxes of whole z rows when z/2 is not a power of two (48^4: 1 x 1 x 8 x 24)
+    if (ext[3] & (ext[3] - 1))
+        for (int n = 64; n <= LT_LINK_BS; n += 64) {
+            if ((threads > 0 && n != threads) || n % ext[3]) continue;
+            const int rest = n / ext[3];
+            for (int ty = 1; ty <= rest; ++ty)
+                for (int tx = 1; tx * ty <= rest; ++tx) {
+                    if (rest % (ty * tx)) continue;
+                    const int tt = rest / (ty * tx);
+                    LinkMap m{};
+                    if (make_linkmap(g, tt, tx, ty, ext[3], m)) out.push_back(m);
+                }
+        }
     // linear maps of consecutive checkerboard sites
     for (int n : {64, 128, 256}) {
         LinkMap m{};
         if ((threads <= 0 || n == threads) && make_linkmap(g, 0, 0, 0, n, m)) out.push_back(m);
     }
-    if (out.empty()) out.push_back(default_linkmap(g));
+    const LinkMap d = default_linkmap(g);
+    bool have = false;
+    for (const LinkMap& m : out) have = have || same_map(m, d);
+    if (!have && (threads <= 0 || d.nthreads == threads)) out.push_back(d);
+    if (out.empty()) out.push_back(d);
+    return out;
+}
+
+// Site orders of one block shape: t tiles per chunk (1, or one t-slab per XCD), y tiles
+// per chunk (1, 2 or all) for the family's kernels (Jacobi: yc, colour sweeps: ycs), and
+// the block index spread over the XCDs or not.  Every order gives the same results.
+std::vector<LinkMap> order_candidates(const LinkMap& base, bool sweeps)
+{
+    std::vector<LinkMap> out;
+    if (base.lin) {   // a linear map only chooses the XCD spread
+        for (int xr : {1, 0}) {
+            LinkMap m = base;
+            m.xr = xr;
+            out.push_back(m);
+        }
+        return out;
+    }
+    std::vector<int> tcs{1}, ycs{1};
+    if (base.N[0] % 8 == 0 && base.N[0] / 8 > 1) tcs.push_back(base.N[0] / 8);
+    if (base.N[2] % 2 == 0 && base.N[2] > 2) ycs.push_back(2);
+    if (base.N[2] > 1) ycs.push_back(base.N[2]);
+    for (int xr : {1, 0})
+        for (int tc : tcs)
+            for (int yc : ycs) {
+                LinkMap m = base;
+                m.xr = xr;
+                m.tc = tc;
+                (sweeps ? m.ycs : m.yc) = yc;
+                out.push_back(m);
+            }
     return out;
 }
 
@@ -61,11 +108,12 @@ std::string tune_key(int device, const Geom& g, const char* family)
     return s.str();
 }
 
-// Cache file: one JSON object, "key": [tt, tx, ty, tz, ms] per entry (tt = tx = ty = 0: a
-// linear map of tz consecutive checkerboard sites per block).
-static std::map<std::string, std::array<double, 5>> read_cache(const char* path)
+// Cache file: one JSON object, "key": [tt, tx, ty, tz, ms, tc, yc, ycs, xr] per entry (tt =
+// tx = ty = 0: a linear map of tz consecutive checkerboard sites per block; tc, yc, ycs, xr:
+// the site order, see order_candidates; entries of five numbers keep the default order).
+static std::map<std::string, std::array<double, 9>> read_cache(const char* path)
 {
-    std::map<std::string, std::array<double, 5>> m;
+    std::map<std::string, std::array<double, 9>> m;
     if (!path || !*path) return m;
     std::ifstream f(path);
     if (!f) return m;
@@ -79,18 +127,19 @@ static std::map<std::string, std::array<double, 5>> read_cache(const char* path)
         const std::string key = s.substr(i + 1, j - i - 1);
         const size_t a = s.find('[', j), b = s.find(']', j);
         if (a == std::string::npos || b == std::string::npos || b < a) break;
-        std::array<double, 5> v{};
+        std::array<double, 9> v{};
         std::istringstream in(s.substr(a + 1, b - a - 1));
         std::string tok;
         int n = 0;
-        while (n < 5 && std::getline(in, tok, ',')) v[n++] = std::atof(tok.c_str());
-        if (n == 5) m[key] = v;
+        v[5] = v[6] = v[7] = v[8] = -1.0;   // entries of five numbers: the map's default order
+        while (n < 9 && std::getline(in, tok, ',')) v[n++] = std::atof(tok.c_str());
+        if (n >= 5) m[key] = v;
         i = b + 1;
     }
     return m;
 }
 
-static void write_cache(const char* path, const std::map<std::string, std::array<double, 5>>& m)
+static void write_cache(const char* path, const std::map<std::string, std::array<double, 9>>& m)
 {
     if (!path || !*path) return;
     std::ofstream f(path, std::ios::trunc);
@@ -98,9 +147,10 @@ static void write_cache(const char* path, const std::map<std::string, std::array
     f << "{\n";
     size_t k = 0;
     for (const auto& e : m) {
-        char buf[160];
-        std::snprintf(buf, sizeof buf, "[%d, %d, %d, %d, %.6g]", (int)e.second[0], (int)e.second[1],
-                      (int)e.second[2], (int)e.second[3], e.second[4]);
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "[%d, %d, %d, %d, %.6g, %d, %d, %d, %d]", (int)e.second[0], (int)e.second[1],
+                      (int)e.second[2], (int)e.second[3], e.second[4], (int)e.second[5], (int)e.second[6],
+                      (int)e.second[7], (int)e.second[8]);
         f << "  \"" << e.first << "\": " << buf << (++k < m.size() ? ",\n" : "\n");
     }
     f << "}\n";
@@ -113,6 +163,15 @@ bool cached_tile(const char* path, const std::string& key, const Geom& g, LinkMa
     if (it == c.end()) return false;
     const auto& v = it->second;
     if (!make_linkmap(g, (int)v[0], (int)v[1], (int)v[2], (int)v[3], m)) return false;
+    if (v[5] >= 0.0) {   // site order: t chunk, y chunks (Jacobi, sweeps), XCD spread
+        LinkMap o = m;
+        o.tc = (int)v[5];
+        o.yc = (int)v[6];
+        o.ycs = (int)v[7];
+        o.xr = (int)v[8];
+        if (o.tc < 1 || o.N[0] % o.tc || o.yc < 1 || o.N[2] % o.yc || o.ycs < 1 || o.N[2] % o.ycs) return false;
+        m = o;
+    }
     ms = v[4];
     return true;
 }
@@ -121,34 +180,56 @@ void store_tile(const char* path, const std::string& key, const LinkMap& m, doub
 {
     if (!path || !*path) return;
     auto c = read_cache(path);
-    c[key] = {m.lin ? 0.0 : (double)m.T[0], m.lin ? 0.0 : (double)m.T[1], m.lin ? 0.0 : (double)m.T[2], (double)m.T[3], ms};
+    c[key] = {m.lin ? 0.0 : (double)m.T[0], m.lin ? 0.0 : (double)m.T[1], m.lin ? 0.0 : (double)m.T[2], (double)m.T[3], ms,
+              (double)m.tc, (double)m.yc, (double)m.ycs, (double)m.xr};
     write_cache(path, c);
 }
 
-LinkMap tune(lt_ctx ctx, const std::vector<LinkMap>& cands, const std::function<void(const LinkMap&)>& run,
-             double& best_ms, int reps)
+// One timing of a few milliseconds varies by a few per cent with the clock the GPU runs
+// at, so the first pass only ranks the candidates: the fastest four and the incumbent are
+// then timed again in three alternating rounds, and each keeps its best time.
+LinkMap tune(lt_ctx ctx, const std::vector<LinkMap>& cands, const LinkMap& incumbent,
+             const std::function<void(const LinkMap&)>& run, double& best_ms, int reps)
 {
     hipEvent_t e0, e1;
     LT_HIP_CHECK(hipEventCreate(&e0));
     LT_HIP_CHECK(hipEventCreate(&e1));
     const bool prof = ctx->profiling;
     ctx->profiling = false;
-    LinkMap best = cands.front();
+    LinkMap best = incumbent;
     best_ms = 1e30;
+    auto time_one = [&](const LinkMap& m) {
+        LT_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; ++r) run(m);
+        LT_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+        LT_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        LT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / reps;
+    };
     try {
-        for (const LinkMap& m : cands) {
-            run(m);   // warm-up
-            LT_HIP_CHECK(hipEventRecord(e0, ctx->stream));
-            for (int r = 0; r < reps; ++r) run(m);
-            LT_HIP_CHECK(hipEventRecord(e1, ctx->stream));
-            LT_HIP_CHECK(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            LT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms / reps < best_ms) {
-                best_ms = ms / reps;
-                best = m;
-            }
+        for (int r = 0; r < 3; ++r) run(incumbent);   // clocks up before the first timing
+        std::vector<std::pair<double, size_t>> rank;
+        for (size_t i = 0; i < cands.size(); ++i) {
+            run(cands[i]);   // warm-up
+            rank.emplace_back(time_one(cands[i]), i);
         }
+        std::sort(rank.begin(), rank.end());
+        std::vector<LinkMap> fin{incumbent};
+        for (size_t k = 0; k < rank.size() && fin.size() < 5; ++k)
+            if (!same_map(cands[rank[k].second], incumbent)) fin.push_back(cands[rank[k].second]);
+        std::vector<double> t(fin.size(), 1e30);
+        for (int round = 0; round < 3; ++round)
+            for (size_t i = 0; i < fin.size(); ++i) {
+                run(fin[i]);
+                t[i] = std::min(t[i], time_one(fin[i]));
+            }
+        size_t w = 0;
+        for (size_t i = 1; i < fin.size(); ++i)
+            if (t[i] < t[w]) w = i;
+        if (t[w] > 0.98 * t[0]) w = 0;
+        best = fin[w];
+        best_ms = t[w];
     } catch (...) {
         ctx->profiling = prof;
         (void)hipEventDestroy(e0);
@@ -196,17 +277,24 @@ extern "C" int lt_autotune_field(lt_field f, int family, const char* cache_path,
                 lt_field_s tmp = *f;
                 tmp.d = a;
                 tmp.view = true;
+                // block shape first (default site order), then the site order of the best
+                // shape; the field's default map is the incumbent
+                const lt::LinkMap d = lt::default_linkmap(f->g);
                 if (family == LT_TUNE_SWEEP) {
-                    m = lt::tune(ctx, lt::tile_candidates(f->g, 0), [&](const lt::LinkMap& c) {
+                    auto run = [&](const lt::LinkMap& c) {
                         tmp.map = c;
                         lt::k_or_sweep(&tmp, 1, LT_CB2);
-                    }, ms, 3);
+                    };
+                    m = lt::tune(ctx, lt::tile_candidates(f->g, 0), d, run, ms, 3);
+                    m = lt::tune(ctx, lt::order_candidates(m, true), m, run, ms, 3);
                 } else {
                     b = scratch_links(nl);
                     z = scratch_links((size_t)32 * f->g.Vh);
-                    m = lt::tune(ctx, lt::tile_candidates(f->g, 0), [&](const lt::LinkMap& c) {
+                    auto run = [&](const lt::LinkMap& c) {
                         lt::k_flow_stage(ctx, f->g, c, a, b, z, 0.0, 0.01, 0.25, 0, 1);
-                    }, ms, 3);
+                    };
+                    m = lt::tune(ctx, lt::tile_candidates(f->g, 0), d, run, ms, 3);
+                    m = lt::tune(ctx, lt::order_candidates(m, false), m, run, ms, 3);
                 }
             } catch (...) {
                 (void)hipStreamSynchronize(ctx->stream);

@@ -204,14 +204,14 @@ __global__ void LT_LINK_KERNEL staple_field_kernel(const double2* __restrict__ U
     st_link(out, g, s, mu, par, staple(U, g, s, c, mu));
 }
 
-// The colour launches walk all y tiles of an x step in one chunk (LinkMap::yc = N[2]):
-// with the alternating walk (rev) that is 2 % faster per HB + 4 OR update at 32^4 than
-// the two-tile chunks the Jacobi kernels keep (the RK3 flow stage is 1.5 % slower with one
-// chunk).  The tile order never changes a result.
+// The colour launches chunk the y tiles by LinkMap::ycs (by default all y tiles of an x
+// step in one chunk: with the alternating walk (rev) 2 % faster per HB + 4 OR update at
+// 32^4 than the two-tile chunks the Jacobi kernels keep; the RK3 flow stage is 1.5 %
+// slower with one chunk).  The tile order never changes a result.
 LinkMap sweep_map(const LinkMap& m)
 {
     LinkMap r = m;
-    r.yc = r.N[2];
+    r.yc = r.ycs;
     return r;
 }
 
@@ -330,13 +330,6 @@ void drop_update_graph(lt_field f)
 }
 
 namespace {
-
-bool same_map(const LinkMap& a, const LinkMap& b)
-{
-    for (int i = 0; i < 4; ++i)
-        if (a.T[i] != b.T[i] || a.N[i] != b.N[i]) return false;
-    return a.nthreads == b.nthreads && a.tc == b.tc && a.yc == b.yc && a.lin == b.lin;
-}
 
 // the launches of one update, in the order of the eager sweeps
 void enqueue_update(lt_field u, UpdateGraph& G, double prefactor)

@@ -59,13 +59,23 @@ def test_tiles_and_autotune_give_identical_results(la, lto, gpu_ctx, tmp_path):
     keys = sorted(entries)
     assert len(keys) == 2 and keys[0].endswith("|16x16x16x16|flow") and keys[1].endswith("|16x16x16x16|sweep")
     assert tuple(entries[keys[1]][:4]) == got[2] and tuple(entries[keys[0]][:4]) == got[3]
-    # a cached entry is applied as stored
+    assert len(entries[keys[0]]) == 9 and len(entries[keys[1]]) == 9      # shape, time and site order
+    # a cached entry is applied as stored (five numbers: the shape's default site order)
     entries[keys[1]] = [2, 2, 4, 8, 1.0]
     json.dump(entries, open(cache, "w"))
     U = la.GaugeField(gpu_ctx, G16)
     tile, ms = U.autotune("sweep", cache)
     assert tile == (2, 2, 4, 8) and ms == 1.0
     U.close()
+    # site orders (t / y chunks, XCD spread) and a linear map as cached entries: same results
+    for sweep_entry, flow_entry in [([1, 1, 16, 8, 1.0, 1, 1, 1, 0], [1, 1, 16, 8, 1.0, 1, 1, 1, 0]),
+                                    ([1, 2, 4, 8, 1.0, 2, 2, 4, 1], [1, 2, 4, 8, 1.0, 2, 4, 2, 0]),
+                                    ([0, 0, 0, 128, 1.0, 1, 1, 1, 0], [0, 0, 0, 256, 1.0, 1, 1, 1, 1])]:
+        flush(f"site order {sweep_entry} / {flow_entry}")
+        json.dump({keys[1]: sweep_entry, keys[0]: flow_entry}, open(cache, "w"))
+        got = sweep_and_flow(la, gpu_ctx, U0, tune_cache=cache)
+        assert got[2] == tuple(sweep_entry[:4]) and got[3] == tuple(flow_entry[:4])
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
 
 
 def test_batch_autotune_keeps_results(la, lto, gpu_ctx, tmp_path):
