@@ -469,6 +469,9 @@ def main():
     ctx.synchronize()
     cb4_update_ms = (time.perf_counter() - t0) / 3 * 1e3
     # BMW files written and read back through the device codec (#18BMW, all rows)
+    # read ceilings of this GPU: 16-byte lane loads from a 2 MiB buffer (L2-resident on every
+    # XCD) and from a 1 GiB buffer (HBM)
+    read_ceiling = {"l2_GBs": ctx.read_rate(1 << 21) / 1e9, "hbm_GBs": ctx.read_rate(1 << 30) / 1e9}
     io = {}
     try:
         import tempfile
@@ -516,6 +519,7 @@ def main():
         "flow_rk3": {"ms_per_step": flow_ms, "model_GBs": flow_gbs, "stage_avg_launch_ms": stage_ms,
                      "stage_GBs": BYTES_PER_LINK_STAGE * 4 * V / (stage_ms * 1e-3) / 1e9,
                      "link_stage_updates_per_s": 12 * V / (flow_ms * 1e-3)},
+        "read_ceiling": read_ceiling,
         "observables_pass_ms": obs_ms,
         "observables_rest_ms": obs_rest_ms,
         "lw_or_sweep_ms": lw_or_ms,

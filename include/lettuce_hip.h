@@ -65,6 +65,13 @@ int lt_ctx_profile(lt_ctx ctx, int enable);
 int lt_ctx_profile_query(lt_ctx ctx, const char *kernel, int *launches, double *total_ms);
 /* free and total device memory of the context's GPU (hipMemGetInfo), for peak-memory reports */
 int lt_ctx_mem_info(lt_ctx ctx, size_t *free_bytes, size_t *total_bytes);
+/* Measurement helper with no counterpart in the reference: the rate (bytes/s) at which
+ * 16-byte lane loads read a zeroed buffer of footprint_bytes (a power of two in
+ * [64 KiB, 1 GiB]), each wavefront load 1 KiB of consecutive bytes, successive loads
+ * of a lane far apart so that the L1 misses.  A footprint below one XCD's L2 gives the
+ * L2 read ceiling that latency-bound link kernels are compared against.  Allocates and
+ * frees its own buffers; best of five timed launches (HIP events, context stream). */
+int lt_ctx_read_rate(lt_ctx ctx, size_t footprint_bytes, double *bytes_per_s);
 
 /* ---- gauge fields ---------------------------------------------------------
  * GaugeField4D<Nt,Nx,Ny,Nz,Matrix_SU3> (lattice.hpp:87-394) with runtime

@@ -82,6 +82,13 @@ class Context:
         call("lt_ctx_mem_info", self._h, ctypes.byref(f), ctypes.byref(t))
         return f.value, t.value
 
+    def read_rate(self, footprint_bytes=1 << 21):
+        """Bytes/s that 16-byte lane loads read from a buffer of footprint_bytes (a power of two;
+        below one XCD's L2: the L2 read ceiling)."""
+        r = ctypes.c_double()
+        call("lt_ctx_read_rate", self._h, footprint_bytes, ctypes.byref(r))
+        return r.value
+
     def kernel_time(self, name):
         n = ctypes.c_int()
         ms = ctypes.c_double()

@@ -67,6 +67,7 @@ SIGNATURES = {
     "lt_cv_create_checkpointed": [c_ctx, I, I, I, I, I, ctypes.c_double, I, ctypes.POINTER(ctypes.c_void_p)],
     "lt_cv_bytes": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)],
     "lt_ctx_mem_info": [c_ctx, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+    "lt_ctx_read_rate": [c_ctx, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)],
     "lt_cv_destroy": [ctypes.c_void_p],
     "lt_cv_forward": [ctypes.c_void_p, c_field, D],
     "lt_cv_backward": [ctypes.c_void_p, ctypes.c_double, c_field],

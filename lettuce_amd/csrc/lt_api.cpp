@@ -3,6 +3,7 @@
 // path (sweep schedules, smearing levels, flow integrators).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -304,6 +305,55 @@ int lt_ctx_mem_info(lt_ctx ctx, size_t* free_bytes, size_t* total_bytes)
         LT_HIP_CHECK(hipMemGetInfo(&f, &t));
         if (free_bytes) *free_bytes = f;
         if (total_bytes) *total_bytes = t;
+        return LT_OK;
+    });
+}
+
+int lt_ctx_read_rate(lt_ctx ctx, size_t footprint_bytes, double* bytes_per_s)
+{
+    if (!ctx || !bytes_per_s) return fail(LT_EINVAL, "null argument");
+    if (footprint_bytes < ((size_t)1 << 16) || footprint_bytes > ((size_t)1 << 30) ||
+        (footprint_bytes & (footprint_bytes - 1)))
+        return fail(LT_EINVAL, "lt_ctx_read_rate: footprint must be a power of two in [64 KiB, 1 GiB]");
+    return guarded([&] {
+        use_device(ctx);
+        const unsigned nb = 2048, lanes = nb * 256;
+        const int iters = 128;
+        double2 *buf = nullptr, *out = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = [&] {
+            (void)hipStreamSynchronize(ctx->stream);
+            if (buf) (void)hipFree(buf);
+            if (out) (void)hipFree(out);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        };
+        try {
+            LT_HIP_CHECK(hipMalloc(&buf, footprint_bytes));
+            LT_HIP_CHECK(hipMalloc(&out, (size_t)lanes * sizeof(double2)));
+            LT_HIP_CHECK(hipMemsetAsync(buf, 0, footprint_bytes, ctx->stream));
+            LT_HIP_CHECK(hipEventCreate(&e0));
+            LT_HIP_CHECK(hipEventCreate(&e1));
+            const unsigned mask = (unsigned)(footprint_bytes / sizeof(double2)) - 1u;
+            const bool prof = ctx->profiling;
+            ctx->profiling = false;
+            double best = 1e30;
+            for (int r = 0; r < 6; ++r) {   // the first launch warms the caches and the clock
+                LT_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+                lt::k_l2_read(ctx, buf, mask, iters, out, nb);
+                LT_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+                LT_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                LT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) best = std::min(best, (double)ms);
+            }
+            ctx->profiling = prof;
+            *bytes_per_s = (double)lanes * iters * sizeof(double2) / (best * 1e-3);
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
         return LT_OK;
     });
 }

@@ -142,6 +142,26 @@ __global__ void __launch_bounds__(BS) copy_kernel(const double2* __restrict__ sr
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// Read rate of a buffer of mask + 1 16-byte elements (a power of two): every lane reads
+// `iters` elements, a wavefront 1 KiB of consecutive elements per load, and successive
+// loads of a lane jump 1021 KiB ahead (mod the buffer), so the L1 misses and a buffer
+// smaller than an XCD's L2 is read from the L2.  One result per lane keeps the loads.
+__global__ void __launch_bounds__(BS) l2_read_kernel(const double2* __restrict__ buf, unsigned mask, int iters,
+                                                     double2* __restrict__ out)
+{
+    const unsigned tid = blockIdx.x * BS + threadIdx.x;
+    unsigned i = tid;
+    double ax = 0.0, ay = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < iters; ++k) {
+        const double2 v = buf[i & mask];
+        ax += v.x;
+        ay += v.y;
+        i += 64u * 1021u;
+    }
+    out[tid] = make_double2(ax, ay);
+}
+
 __global__ void __launch_bounds__(BS) axpy_kernel(const double2* __restrict__ x, double2* __restrict__ y, double a,
                                                   size_t n)
 {
@@ -175,6 +195,11 @@ void k_scale(lt_ctx ctx, double2* y, double a, size_t n)
     unsigned nb = nblocks((long long)n, BS);
     if (nb > 65536u) nb = 65536u;
     launch(ctx, "scale_links", scale_kernel, dim3(nb), dim3(BS), 0, y, a, n);
+}
+
+void k_l2_read(lt_ctx ctx, const double2* buf, unsigned mask, int iters, double2* out, unsigned nb)
+{
+    launch(ctx, "l2_read", l2_read_kernel, dim3(nb), dim3(BS), 0, buf, mask, iters, out);
 }
 
 void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n)

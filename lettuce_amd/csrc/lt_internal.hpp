@@ -230,6 +230,9 @@ inline bool same_map(const LinkMap& a, const LinkMap& b)
 
 // kernel launchers (defined in the .hip translation units)
 void k_copy(lt_ctx ctx, const double2* src, double2* dst, size_t n);
+// read-rate probe: nb blocks of 256 lanes, each reading iters 16-byte elements of buf
+// (mask + 1 elements, a power of two) and writing one to out (nb * 256 elements)
+void k_l2_read(lt_ctx ctx, const double2* buf, unsigned mask, int iters, double2* out, unsigned nb);
 // y += a x over n complex entries (the bias-force kick P += eps F)
 void k_axpy(lt_ctx ctx, const double2* x, double2* y, double a, size_t n);
 // y *= a over n complex entries (ReverseMomenta with a = -1)

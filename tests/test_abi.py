@@ -46,6 +46,7 @@ def test_null_arguments_are_rejected():
     assert b"null" in lib.lt_last_error()
     assert lib.lt_or_sweep(None, 1, 2) == -1
     assert lib.lt_observables(None, None) == -1
+    assert lib.lt_ctx_read_rate(None, 1 << 21, None) == -1
     # destroying null handles is a no-op
     assert lib.lt_field_destroy(None) == 0
     assert lib.lt_ctx_destroy(None) == 0

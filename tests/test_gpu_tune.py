@@ -1,6 +1,6 @@
 """Launch-shape tuning on the GPU: every thread-block tile gives bit-identical
 results (sweeps, flow, batched MetaD), and the autotuner's JSON cache is written
-and reused."""
+and reused; the read-rate probe behind the roofline ceilings."""
 import json
 
 import numpy as np
@@ -103,3 +103,15 @@ def test_batch_autotune_keeps_results(la, lto, gpu_ctx, tmp_path):
     for a, c in zip(out[0][2], out[1][2]):
         assert np.array_equal(a, c)
     assert len(json.load(open(tmp_path / "batch.json"))) == 2     # one entry per distinct extent
+
+
+def test_read_rate_probe(la, gpu_ctx):
+    """The read-rate probe: an L2-resident buffer reads faster than one that streams from
+    HBM, both rates positive and finite; bad footprints are refused."""
+    flush("read-rate probe")
+    l2 = gpu_ctx.read_rate(1 << 21)
+    hbm = gpu_ctx.read_rate(1 << 28)
+    assert 0.0 < hbm < l2 < 1e15
+    for bad in (3 << 20, 1 << 10, 1 << 31):
+        with pytest.raises(la.LettuceError):
+            gpu_ctx.read_rate(bad)
