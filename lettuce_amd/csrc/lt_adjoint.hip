@@ -48,11 +48,11 @@ __global__ void __launch_bounds__(CBS) sum_kernel(const double* part, int n, dou
     block_sum_to<CBS>(a, out);
 }
 
-template <int HALF, int K>
-__global__ void LT_LINK_KERNEL clover_deriv_kernel(const double2* __restrict__ U, const double2* __restrict__ C,
-                                                   double2* __restrict__ F, Geom g, LinkMap m, double scale_factor)
+__global__ void LT_LINK_KERNEL clover_deriv_all_kernel(const double2* __restrict__ U, const double2* __restrict__ C,
+                                                       double2* __restrict__ F, Geom g, LinkMap m, double scale_factor)
 {
-    body_clover_deriv<HALF, K>(U, C, F, g, m, link_block(m, blockIdx.x, gridDim.x), scale_factor);
+    extern __shared__ double2 acc[];
+    body_clover_deriv_all(U, C, F, g, m, link_block(m, blockIdx.x, gridDim.x), scale_factor, acc);
 }
 
 __global__ void LT_LINK_KERNEL lambda_kernel(const double2* __restrict__ U, const double2* __restrict__ Uprev,
@@ -98,13 +98,12 @@ double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C)
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
                     double scale_factor)
 {
+    // one launch for the six half terms (156 -> 150 us per 16^4 force evaluation); the
+    // replica batches keep the six launches, where the fused kernel's spills cost more
+    // than the five passes over F it saves
     const dim3 grid(8 * map_blocks(m)), block(m.nthreads);
-    launch(ctx, "clover_deriv", clover_deriv_kernel<0, 0>, grid, block, 0, U, C, F, g, m, scale_factor);
-    launch(ctx, "clover_deriv", clover_deriv_kernel<1, 0>, grid, block, 0, U, C, F, g, m, scale_factor);
-    launch(ctx, "clover_deriv", clover_deriv_kernel<0, 1>, grid, block, 0, U, C, F, g, m, scale_factor);
-    launch(ctx, "clover_deriv", clover_deriv_kernel<1, 1>, grid, block, 0, U, C, F, g, m, scale_factor);
-    launch(ctx, "clover_deriv", clover_deriv_kernel<0, 2>, grid, block, 0, U, C, F, g, m, scale_factor);
-    launch(ctx, "clover_deriv", clover_deriv_kernel<1, 2>, grid, block, 0, U, C, F, g, m, scale_factor);
+    launch(ctx, "clover_deriv", clover_deriv_all_kernel, grid, block, 9 * m.nthreads * sizeof(double2), U, C, F, g, m,
+           scale_factor);
 }
 
 void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,

@@ -238,12 +238,24 @@ struct CloverTerm {
     static constexpr double sign = (inversions & 1) ? -1.0 : 1.0;
 };
 
-template <int MU, int NU, int RHO, int SIGMA>
+// neighbour steps in a direction known only at run time (selects instead of indexing
+// the extents, which would put them in scratch)
+__device__ __forceinline__ int up_rt(const Geom& g, int s, int cm, int d)
+{
+    const int L = pick4(g.L, d), st = pick4(g.st, d);
+    return (cm + 1 < L) ? s + st : s - (L - 1) * st;
+}
+__device__ __forceinline__ int dn_rt(const Geom& g, int s, int cm, int d)
+{
+    const int L = pick4(g.L, d), st = pick4(g.st, d);
+    return (cm > 0) ? s - st : s + (L - 1) * st;
+}
+
 __device__ __forceinline__ M3 cderiv_upper(const double2* __restrict__ U, const double2* __restrict__ C, const Geom& g,
-                                           int s, const int c[4], int par)
+                                           int s, const int c[4], int par, int MU, int NU, int RHO, int SIGMA)
 {
     const int q = par ^ 1;
-    const int smu = up(g, s, c[MU], MU), snu = up(g, s, c[NU], NU);
+    const int smu = up_rt(g, s, pick4(c, MU), MU), snu = up_rt(g, s, pick4(c, NU), NU);
     const M3 Unu_x = ldu(U, g, s, NU, par);
     M3 tmp1, tmp2;
     {
@@ -255,18 +267,17 @@ __device__ __forceinline__ M3 cderiv_upper(const double2* __restrict__ U, const 
     S = add(S, mul_nd(R_at(C, g, snu, q, RHO, SIGMA), Unu_x));
     M3 acc = mul(tmp1, S);
     const M3 Unu_mu = ldu(U, g, smu, NU, q);
-    S = mul(Unu_mu, R_at(C, g, up(g, smu, c[NU], NU), par, RHO, SIGMA));
+    S = mul(Unu_mu, R_at(C, g, up_rt(g, smu, pick4(c, NU), NU), par, RHO, SIGMA));
     S = add(S, mul(R_at(C, g, smu, q, RHO, SIGMA), Unu_mu));
     return add(acc, mul(S, tmp2));
 }
 
-template <int MU, int NU, int RHO, int SIGMA>
 __device__ __forceinline__ M3 cderiv_lower(const double2* __restrict__ U, const double2* __restrict__ C, const Geom& g,
-                                           int s, const int c[4], int par)
+                                           int s, const int c[4], int par, int MU, int NU, int RHO, int SIGMA)
 {
     const int q = par ^ 1;
-    const int smu = up(g, s, c[MU], MU);
-    const int snd = dn(g, s, c[NU], NU), smu_nd = dn(g, smu, c[NU], NU);
+    const int smu = up_rt(g, s, pick4(c, MU), MU);
+    const int snd = dn_rt(g, s, pick4(c, NU), NU), smu_nd = dn_rt(g, smu, pick4(c, NU), NU);
     const M3 Unu_nd = ldu(U, g, snd, NU, q);
     const M3 Unu_mund = ldu(U, g, smu_nd, NU, par);
     M3 tmp3, tmp4;
@@ -291,11 +302,11 @@ __device__ __forceinline__ void cderiv_half_mu(const double2* __restrict__ U, co
     using T = CloverTerm<MU, K>;
     M3 acc;
     if (HALF == 0) {
-        const M3 up_part = scale(T::sign, cderiv_upper<MU, T::nu, T::rho, T::sigma>(U, C, g, s, c, par));
+        const M3 up_part = scale(T::sign, cderiv_upper(U, C, g, s, c, par, MU, T::nu, T::rho, T::sigma));
         acc = K == 0 ? up_part : add(ld(F, g, s, MU, par), up_part);
         st_link(F, g, s, MU, par, acc);
     } else {
-        const M3 low = scale(T::sign, cderiv_lower<MU, T::nu, T::rho, T::sigma>(U, C, g, s, c, par));
+        const M3 low = scale(T::sign, cderiv_lower(U, C, g, s, c, par, MU, T::nu, T::rho, T::sigma));
         acc = sub(ld(F, g, s, MU, par), low);
         if (K == 2) {
             const double pref = -2.0 / (512.0 * 3.14159265358979323846 * 3.14159265358979323846);
@@ -319,6 +330,52 @@ __device__ __forceinline__ void body_clover_deriv(const double2* __restrict__ U,
         case 2: cderiv_half_mu<HALF, K, 2>(U, C, F, g, s, c, par, scale_factor); break;
         default: cderiv_half_mu<HALF, K, 3>(U, C, F, g, s, c, par, scale_factor); break;
     }
+}
+
+// All six half terms of link (x, mu) in one pass, in the order of the six launches
+// above (upper and lower half of the k-th direction nu != mu, k = 0, 1, 2): the same
+// operations in the same order, so the same F bit for bit, without five reads and
+// writes of F.  The running sum waits in LDS (acc: 9 x nthreads entries, entry
+// e * nthreads + lane), so that each half has all registers to itself; the empty asm
+// with a memory clobber keeps the compiler from carrying it in registers instead.  The
+// directions are loop variables (one copy of the code for every mu and k).
+__device__ __forceinline__ void lds_acc(double2* acc, int n, const M3& a, int first, int negate)
+{
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        const cd v = a.m[e];
+        if (first) {
+            acc[e * n] = make_double2(v.re, v.im);
+        } else {
+            const double2 o = acc[e * n];
+            acc[e * n] = negate ? make_double2(o.x - v.re, o.y - v.im) : make_double2(o.x + v.re, o.y + v.im);
+        }
+    }
+    asm volatile("" ::: "memory");
+}
+__device__ __forceinline__ void body_clover_deriv_all(const double2* __restrict__ U, const double2* __restrict__ C,
+                                                      double2* __restrict__ F, const Geom& g, const LinkMap& m, int b,
+                                                      double scale_factor, double2* acc)
+{
+    int mu, par, c[4], s;
+    if (!map_link_b(g, m, b, mu, par, c, s)) return;
+    acc += threadIdx.x;
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {
+        const int nu = k < mu ? k : k + 1;
+        const int rho = (mu != 0 && nu != 0) ? 0 : ((mu != 1 && nu != 1) ? 1 : 2);
+        const int sigma = (mu != 3 && nu != 3) ? 3 : ((mu != 2 && nu != 2) ? 2 : 1);
+        const int inv = (mu > nu) + (mu > rho) + (mu > sigma) + (nu > rho) + (nu > sigma) + (rho > sigma);
+        const double sign = (inv & 1) ? -1.0 : 1.0;
+        lds_acc(acc, m.nthreads, scale(sign, cderiv_upper(U, C, g, s, c, par, mu, nu, rho, sigma)), k == 0, 0);
+        lds_acc(acc, m.nthreads, scale(sign, cderiv_lower(U, C, g, s, c, par, mu, nu, rho, sigma)), 0, 1);
+    }
+    M3 a;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a.m[e] = cd{acc[e * m.nthreads].x, acc[e * m.nthreads].y};
+    const double pref = -2.0 / (512.0 * 3.14159265358979323846 * 3.14159265358979323846);
+    a = scale(scale_factor * pref, algebra(mul(ldu(U, g, s, mu, par), a)));
+    st_link(F, g, s, mu, par, a);
 }
 
 // Sigma' = Uprev^+ Sigma (stored in place) and
