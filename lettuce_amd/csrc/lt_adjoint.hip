@@ -98,11 +98,8 @@ double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C)
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
                     double scale_factor)
 {
-    // one launch for the six half terms (156 -> 150 us per 16^4 force evaluation); the
-    // replica batches keep the six launches, where the fused kernel's spills cost more
-    // than the five passes over F it saves
     const dim3 grid(8 * map_blocks(m)), block(m.nthreads);
-    launch(ctx, "clover_deriv", clover_deriv_all_kernel, grid, block, 9 * m.nthreads * sizeof(double2), U, C, F, g, m,
+    launch(ctx, "clover_deriv", clover_deriv_all_kernel, grid, block, 18 * m.nthreads * sizeof(double2), U, C, F, g, m,
            scale_factor);
 }
 

@@ -72,12 +72,12 @@ __global__ void LT_LINK_KERNEL bk_stout_tape(const double2* __restrict__ Uin, do
     body_stout_tape(Uin + off, Uout + off, T + alg_offset(off), g, m, lb, rho);
 }
 
-template <int HALF, int K>
-__global__ void LT_LINK_KERNEL bk_clover_deriv(const double2* __restrict__ U, const double2* __restrict__ C,
-                                               double2* __restrict__ F, BatchGrid bg, const double* __restrict__ scale)
+__global__ void LT_LINK_KERNEL bk_clover_deriv_all(const double2* __restrict__ U, const double2* __restrict__ C,
+                                                   double2* __restrict__ F, BatchGrid bg, const double* __restrict__ scale)
 {
+    extern __shared__ double2 acc[];
     LT_BATCH_LINK_PROLOGUE
-    body_clover_deriv<HALF, K>(U + off, C + clover_offset(off), F + off, g, m, lb, scale[r]);
+    body_clover_deriv_all(U + off, C + clover_offset(off), F + off, g, m, lb, scale[r], acc);
 }
 
 __global__ void LT_LINK_KERNEL bk_lambda(const double2* __restrict__ U, const double2* __restrict__ Uprev,
@@ -394,12 +394,7 @@ void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double
                      const double* scale)
 {
     const dim3 grid(total_lb), block(link_threads(bg, total_lb));
-    launch(ctx, "batch_clover_deriv", bk_clover_deriv<0, 0>, grid, block, 0, U, C, F, bg, scale);
-    launch(ctx, "batch_clover_deriv", bk_clover_deriv<1, 0>, grid, block, 0, U, C, F, bg, scale);
-    launch(ctx, "batch_clover_deriv", bk_clover_deriv<0, 1>, grid, block, 0, U, C, F, bg, scale);
-    launch(ctx, "batch_clover_deriv", bk_clover_deriv<1, 1>, grid, block, 0, U, C, F, bg, scale);
-    launch(ctx, "batch_clover_deriv", bk_clover_deriv<0, 2>, grid, block, 0, U, C, F, bg, scale);
-    launch(ctx, "batch_clover_deriv", bk_clover_deriv<1, 2>, grid, block, 0, U, C, F, bg, scale);
+    launch(ctx, "batch_clover_deriv", bk_clover_deriv_all, grid, block, 18 * block.x * sizeof(double2), U, C, F, bg, scale);
 }
 
 void kb_stout_recursion(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* Uprev,

@@ -223,20 +223,9 @@ __device__ __forceinline__ double body_qdensity_site(const double2* __restrict__
 //      tmp3 = U_nu(x+mu-nu)^+ U_mu(x-nu)^+, tmp4 = U_mu(x-nu)^+ U_nu(x-nu),
 // accumulate sum_nu s (upper - lower) and end with
 //   F(x, mu) = scale * (-2/(512 pi^2)) Algebra(U_mu(x) * sum).
-// One half-term alone needs ~210 VGPRs; the whole sum in one thread spills, so the
-// derivative runs as six launches (term k = 0..2, upper then lower) that
-// accumulate in F, the last one applying the final projection.
-
-// nu = k-th direction != mu, (rho, sigma) the remaining two in ascending order,
-// sign = eps(mu, nu, rho, sigma)
-template <int MU, int K>
-struct CloverTerm {
-    static constexpr int nu = K < MU ? K : K + 1;
-    static constexpr int rho = (MU != 0 && nu != 0) ? 0 : ((MU != 1 && nu != 1) ? 1 : 2);
-    static constexpr int sigma = (MU != 3 && nu != 3) ? 3 : ((MU != 2 && nu != 2) ? 2 : 1);
-    static constexpr int inversions = (MU > nu) + (MU > rho) + (MU > sigma) + (nu > rho) + (nu > sigma) + (rho > sigma);
-    static constexpr double sign = (inversions & 1) ? -1.0 : 1.0;
-};
+// One half-term alone needs ~210 VGPRs, so the whole sum straight through one thread
+// spills; body_clover_deriv_all runs the halves as a loop with the running sum and one
+// partial product parked in LDS.
 
 // neighbour steps in a direction known only at run time (selects instead of indexing
 // the extents, which would put them in scratch)
@@ -251,8 +240,27 @@ __device__ __forceinline__ int dn_rt(const Geom& g, int s, int cm, int d)
     return (cm > 0) ? s - st : s + (L - 1) * st;
 }
 
+// Parking slot: the first of the two products of a half waits in LDS (9 entries,
+// stride n) while the second is formed, so it holds no registers meanwhile; the empty
+// asm statements with memory clobbers keep the compiler from moving it back.
+__device__ __forceinline__ void park_store(double2* park, int n, const M3& a)
+{
+#pragma unroll
+    for (int e = 0; e < 9; ++e) park[e * n] = make_double2(a.m[e].re, a.m[e].im);
+    asm volatile("" ::: "memory");
+}
+__device__ __forceinline__ M3 park_load(const double2* park, int n)
+{
+    asm volatile("" ::: "memory");
+    M3 a;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a.m[e] = cd{park[e * n].x, park[e * n].y};
+    return a;
+}
+
 __device__ __forceinline__ M3 cderiv_upper(const double2* __restrict__ U, const double2* __restrict__ C, const Geom& g,
-                                           int s, const int c[4], int par, int MU, int NU, int RHO, int SIGMA)
+                                           int s, const int c[4], int par, int MU, int NU, int RHO, int SIGMA,
+                                           double2* park, int n)
 {
     const int q = par ^ 1;
     const int smu = up_rt(g, s, pick4(c, MU), MU), snu = up_rt(g, s, pick4(c, NU), NU);
@@ -266,14 +274,18 @@ __device__ __forceinline__ M3 cderiv_upper(const double2* __restrict__ U, const 
     M3 S = mul_dn(Unu_x, R_at(C, g, s, par, RHO, SIGMA));
     S = add(S, mul_nd(R_at(C, g, snu, q, RHO, SIGMA), Unu_x));
     M3 acc = mul(tmp1, S);
+    park_store(park, n, acc);
     const M3 Unu_mu = ldu(U, g, smu, NU, q);
     S = mul(Unu_mu, R_at(C, g, up_rt(g, smu, pick4(c, NU), NU), par, RHO, SIGMA));
     S = add(S, mul(R_at(C, g, smu, q, RHO, SIGMA), Unu_mu));
-    return add(acc, mul(S, tmp2));
+    const M3 second = mul(S, tmp2);
+    acc = park_load(park, n);
+    return add(acc, second);
 }
 
 __device__ __forceinline__ M3 cderiv_lower(const double2* __restrict__ U, const double2* __restrict__ C, const Geom& g,
-                                           int s, const int c[4], int par, int MU, int NU, int RHO, int SIGMA)
+                                           int s, const int c[4], int par, int MU, int NU, int RHO, int SIGMA,
+                                           double2* park, int n)
 {
     const int q = par ^ 1;
     const int smu = up_rt(g, s, pick4(c, MU), MU);
@@ -288,57 +300,22 @@ __device__ __forceinline__ M3 cderiv_lower(const double2* __restrict__ U, const 
     }
     M3 S = mul(Unu_nd, R_at(C, g, s, par, RHO, SIGMA));
     S = add(S, mul(R_at(C, g, snd, q, RHO, SIGMA), Unu_nd));
-    const M3 acc = mul(tmp3, S);
+    M3 acc = mul(tmp3, S);
+    park_store(park, n, acc);
     S = mul_dn(Unu_mund, R_at(C, g, smu_nd, par, RHO, SIGMA));
     S = add(S, mul_nd(R_at(C, g, smu, q, RHO, SIGMA), Unu_mund));
-    return add(acc, mul(S, tmp4));
+    const M3 second = mul(S, tmp4);
+    acc = park_load(park, n);
+    return add(acc, second);
 }
 
-template <int HALF, int K, int MU>
-__device__ __forceinline__ void cderiv_half_mu(const double2* __restrict__ U, const double2* __restrict__ C,
-                                               double2* __restrict__ F, const Geom& g, int s, const int c[4], int par,
-                                               double scale_factor)
-{
-    using T = CloverTerm<MU, K>;
-    M3 acc;
-    if (HALF == 0) {
-        const M3 up_part = scale(T::sign, cderiv_upper(U, C, g, s, c, par, MU, T::nu, T::rho, T::sigma));
-        acc = K == 0 ? up_part : add(ld(F, g, s, MU, par), up_part);
-        st_link(F, g, s, MU, par, acc);
-    } else {
-        const M3 low = scale(T::sign, cderiv_lower(U, C, g, s, c, par, MU, T::nu, T::rho, T::sigma));
-        acc = sub(ld(F, g, s, MU, par), low);
-        if (K == 2) {
-            const double pref = -2.0 / (512.0 * 3.14159265358979323846 * 3.14159265358979323846);
-            acc = scale(scale_factor * pref, algebra(mul(ldu(U, g, s, MU, par), acc)));
-        }
-        st_link(F, g, s, MU, par, acc);
-    }
-}
-
-// one of the six launches: half HALF (0 upper, 1 lower) of term K
-template <int HALF, int K>
-__device__ __forceinline__ void body_clover_deriv(const double2* __restrict__ U, const double2* __restrict__ C,
-                                                  double2* __restrict__ F, const Geom& g, const LinkMap& m, int b,
-                                                  double scale_factor)
-{
-    int mu, par, c[4], s;
-    if (!map_link_b(g, m, b, mu, par, c, s)) return;
-    switch (mu) {
-        case 0: cderiv_half_mu<HALF, K, 0>(U, C, F, g, s, c, par, scale_factor); break;
-        case 1: cderiv_half_mu<HALF, K, 1>(U, C, F, g, s, c, par, scale_factor); break;
-        case 2: cderiv_half_mu<HALF, K, 2>(U, C, F, g, s, c, par, scale_factor); break;
-        default: cderiv_half_mu<HALF, K, 3>(U, C, F, g, s, c, par, scale_factor); break;
-    }
-}
-
-// All six half terms of link (x, mu) in one pass, in the order of the six launches
-// above (upper and lower half of the k-th direction nu != mu, k = 0, 1, 2): the same
-// operations in the same order, so the same F bit for bit, without five reads and
-// writes of F.  The running sum waits in LDS (acc: 9 x nthreads entries, entry
-// e * nthreads + lane), so that each half has all registers to itself; the empty asm
-// with a memory clobber keeps the compiler from carrying it in registers instead.  The
-// directions are loop variables (one copy of the code for every mu and k).
+// All six half terms of link (x, mu) in one pass: upper then lower half of the k-th
+// direction nu != mu, k = 0, 1, 2, summed in that order.  The running sum waits in LDS
+// (acc: entries e * nthreads + lane, e < 9) and the first product of each half in the
+// parking slot after it (acc + 9 * nthreads), so that each half has the registers to
+// itself; the empty asm statements with memory clobbers keep the compiler from carrying
+// them in registers instead.  The directions are loop variables (one copy of the code
+// for every mu and k).  LDS: 18 x 16 bytes per lane of the block.
 __device__ __forceinline__ void lds_acc(double2* acc, int n, const M3& a, int first, int negate)
 {
 #pragma unroll
@@ -367,8 +344,11 @@ __device__ __forceinline__ void body_clover_deriv_all(const double2* __restrict_
         const int sigma = (mu != 3 && nu != 3) ? 3 : ((mu != 2 && nu != 2) ? 2 : 1);
         const int inv = (mu > nu) + (mu > rho) + (mu > sigma) + (nu > rho) + (nu > sigma) + (rho > sigma);
         const double sign = (inv & 1) ? -1.0 : 1.0;
-        lds_acc(acc, m.nthreads, scale(sign, cderiv_upper(U, C, g, s, c, par, mu, nu, rho, sigma)), k == 0, 0);
-        lds_acc(acc, m.nthreads, scale(sign, cderiv_lower(U, C, g, s, c, par, mu, nu, rho, sigma)), 0, 1);
+        double2* park = acc + 9 * m.nthreads;
+        lds_acc(acc, m.nthreads, scale(sign, cderiv_upper(U, C, g, s, c, par, mu, nu, rho, sigma, park, m.nthreads)),
+                k == 0, 0);
+        lds_acc(acc, m.nthreads, scale(sign, cderiv_lower(U, C, g, s, c, par, mu, nu, rho, sigma, park, m.nthreads)), 0,
+                1);
     }
     M3 a;
 #pragma unroll
