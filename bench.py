@@ -201,8 +201,8 @@ def metad16(la, ctx, n_traj=3, n_step=20):
     ctx.synchronize()
     n_eval = 2 * n_step + 1
     kernels = {}
-    for name in ("stout_tape", "clover_store", "qdensity", "clover_deriv", "stout_lambda", "stout_sigma_fs",
-                 "stout_sigma", "force_update", "field_update"):
+    for name in ("stout_tape", "clover_store", "qdensity", "clover_deriv", "stout_lambda", "stout_sigma",
+                 "force_update", "field_update"):
         n, ms = ctx.kernel_time(name)
         if n:
             kernels[name] = {"launches_per_eval": n / n_eval, "us_per_eval": 1e3 * ms / n_eval}

@@ -9,7 +9,6 @@
 //   clover    6 planes per site, R = C - C^+ in 9 reals: C[((plane*2 + parity)*5 + k)*Vh + cb]
 //   sigma     force being propagated (in place, level by level)
 //   lambda    per-link Lambda of the current level, 8 reals (ld_her8)
-//   fsum      per-link six-term Lambda staple sum of the current level
 // The per-thread bodies live in lt_bodies.hpp and are shared with the
 // replica-batched kernels.
 #include "lt_internal.hpp"
@@ -62,17 +61,12 @@ __global__ void LT_LINK_KERNEL lambda_kernel(const double2* __restrict__ U, cons
     body_lambda(U, Uprev, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x));
 }
 
-__global__ void LT_LINK_KERNEL sigma_fs_kernel(const double2* __restrict__ U, const double2* __restrict__ Lam,
-                                               double2* __restrict__ Fs, Geom g, LinkMap m)
+__global__ void LT_LINK_KERNEL sigma_all_kernel(const double2* __restrict__ U, double2* __restrict__ Sig,
+                                                const double2* __restrict__ T, const double2* __restrict__ Lam, Geom g,
+                                                LinkMap m, double rho)
 {
-    body_sigma_fs(U, Lam, Fs, g, m, link_block(m, blockIdx.x, gridDim.x));
-}
-
-__global__ void LT_LINK_KERNEL sigma_kernel(const double2* __restrict__ U, double2* __restrict__ Sig,
-                                            const double2* __restrict__ T, const double2* __restrict__ Lam,
-                                            const double2* __restrict__ Fs, Geom g, LinkMap m, double rho)
-{
-    body_sigma(U, Sig, T, Lam, Fs, g, m, link_block(m, blockIdx.x, gridDim.x), rho);
+    extern __shared__ double2 lds[];
+    body_sigma_all(U, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x), rho, lds);
 }
 
 }  // namespace
@@ -104,13 +98,12 @@ void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* 
 }
 
 void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,
-                       double2* Sig, const double2* T, double2* Lam, double2* Fs, double rho)
+                       double2* Sig, const double2* T, double2* Lam, double rho)
 {
     const dim3 grid(8 * map_blocks(m)), block(m.nthreads);
     launch(ctx, "stout_lambda", lambda_kernel, grid, block, 0, U, Uprev, Sig, T, Lam, g, m);
-    launch(ctx, "stout_sigma_fs", sigma_fs_kernel, grid, block, 0, U, (const double2*)Lam, Fs, g, m);
-    launch(ctx, "stout_sigma", sigma_kernel, grid, block, 0, U, Sig, T, (const double2*)Lam, (const double2*)Fs, g, m,
-           rho);
+    launch(ctx, "stout_sigma", sigma_all_kernel, grid, block, 18 * m.nthreads * sizeof(double2), U, Sig, T,
+           (const double2*)Lam, g, m, rho);
 }
 
 }  // namespace lt

@@ -88,19 +88,13 @@ __global__ void LT_LINK_KERNEL bk_lambda(const double2* __restrict__ U, const do
     body_lambda(U + off, Uprev + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), g, m, lb);
 }
 
-__global__ void LT_LINK_KERNEL bk_sigma_fs(const double2* __restrict__ U, const double2* __restrict__ Lam,
-                                           double2* __restrict__ Fs, BatchGrid bg)
+__global__ void LT_LINK_KERNEL bk_sigma_all(const double2* __restrict__ U, double2* __restrict__ Sig,
+                                            const double2* __restrict__ T, const double2* __restrict__ Lam, BatchGrid bg,
+                                            double rho)
 {
+    extern __shared__ double2 lds[];
     LT_BATCH_LINK_PROLOGUE
-    body_sigma_fs(U + off, Lam + alg_offset(off), Fs + off, g, m, lb);
-}
-
-__global__ void LT_LINK_KERNEL bk_sigma(const double2* __restrict__ U, double2* __restrict__ Sig,
-                                        const double2* __restrict__ T, const double2* __restrict__ Lam,
-                                        const double2* __restrict__ Fs, BatchGrid bg, double rho)
-{
-    LT_BATCH_LINK_PROLOGUE
-    body_sigma(U + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), Fs + off, g, m, lb, rho);
+    body_sigma_all(U + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), g, m, lb, rho, lds);
 }
 
 #undef LT_BATCH_LINK_PROLOGUE
@@ -398,13 +392,12 @@ void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double
 }
 
 void kb_stout_recursion(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* Uprev,
-                        double2* Sig, const double2* T, double2* Lam, double2* Fs, double rho)
+                        double2* Sig, const double2* T, double2* Lam, double rho)
 {
     const dim3 grid(total_lb), block(link_threads(bg, total_lb));
     launch(ctx, "batch_stout_lambda", bk_lambda, grid, block, 0, U, Uprev, Sig, T, Lam, bg);
-    launch(ctx, "batch_stout_sigma_fs", bk_sigma_fs, grid, block, 0, U, (const double2*)Lam, Fs, bg);
-    launch(ctx, "batch_stout_sigma", bk_sigma, grid, block, 0, U, Sig, T, (const double2*)Lam, (const double2*)Fs, bg,
-           rho);
+    launch(ctx, "batch_stout_sigma", bk_sigma_all, grid, block, 18 * block.x * sizeof(double2), U, Sig, T,
+           (const double2*)Lam, bg, rho);
 }
 
 void kb_plaquette(lt_ctx ctx, const BatchGrid& bg, int total_sb, const double2* U, double* part, double* out)

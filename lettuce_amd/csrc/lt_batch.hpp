@@ -78,7 +78,7 @@ void kb_clover_charge(lt_ctx ctx, const BatchGrid& bg, int total_sb, const doubl
 void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* C, double2* F,
                      const double* scale);
 void kb_stout_recursion(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* Uprev,
-                        double2* Sig, const double2* T, double2* Lam, double2* Fs, double rho);
+                        double2* Sig, const double2* T, double2* Lam, double rho);
 void kb_plaquette(lt_ctx ctx, const BatchGrid& bg, int total_sb, const double2* U, double* part, double* out);
 void kb_kinetic(lt_ctx ctx, const BatchGrid& bg, int total_kb, const double2* P, double* part, double* out);
 // scalar stages (one thread per replica)

@@ -57,7 +57,6 @@ struct lt_batch_s {
     lt::BatchRep* d_reps = nullptr;
     int* d_pre = nullptr;
     double2 *U = nullptr, *Ucopy = nullptr, *P = nullptr, *F = nullptr, *lambda = nullptr, *clover = nullptr;
-    double2* fsum = nullptr;
     std::vector<double2*> levels;          // levels[0] aliases U
     std::vector<double2*> tapes;
     double* part = nullptr;
@@ -106,7 +105,7 @@ struct lt_batch_s {
         levels.clear();
         tapes.clear();
         for (void* p : {(void*)d_reps, (void*)d_pre, (void*)U, (void*)Ucopy, (void*)P, (void*)F, (void*)lambda,
-                        (void*)fsum, (void*)clover, (void*)part, (void*)state_mem, (void*)st.samples,
+                        (void*)clover, (void*)part, (void*)state_mem, (void*)st.samples,
                         (void*)bias.grid})
             if (p) (void)hipFree(p);
     }
@@ -170,7 +169,7 @@ void enqueue_step(lt_batch b, const StepKey& p, int n_sub)
                 lt::kb_clover_deriv(ctx, bg, b->total_lb, b->levels[b->n_levels], b->clover, b->F, st.dvdq);
                 for (int k = b->n_levels; k > 0; --k)
                     lt::kb_stout_recursion(ctx, bg, b->total_lb, b->levels[k - 1], b->levels[k], b->F,
-                                           b->tapes[k - 1], b->lambda, b->fsum, b->rho);
+                                           b->tapes[k - 1], b->lambda, b->rho);
             }
             if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)
                 lt::kb_force_update(ctx, bg, b->total_lb, b->U, b->P, b->F, e * p.beta / 6.0, e);
@@ -259,7 +258,6 @@ int lt_batch_create(lt_ctx ctx, int n_rep, const int* shapes, int n_levels, doub
             b->P = dev_alloc<double2>(nl);
             b->F = dev_alloc<double2>(nl);
             b->lambda = dev_alloc<double2>((size_t)lt::alg_offset(b->total));
-            b->fsum = dev_alloc<double2>(nl);
             b->clover = dev_alloc<double2>((size_t)lt::clover_offset(b->total));
             b->levels.push_back(b->U);
             for (int l = 0; l < n_levels; ++l) {
@@ -498,8 +496,7 @@ int lt_batch_autotune(lt_batch b, const char* cache_path)
                     double2* tape = b->n_levels > 0 ? b->tapes[0] : b->lambda;
                     if (b->n_levels > 0)   // never writes level 0 (the replicas' fields)
                         lt::kb_stout_tape(b->ctx, bg, b->total_lb, b->levels[0], b->levels[1], tape, b->rho);
-                    lt::kb_stout_recursion(b->ctx, bg, b->total_lb, b->U, b->U, b->F, tape, b->lambda, b->fsum,
-                                           b->rho);
+                    lt::kb_stout_recursion(b->ctx, bg, b->total_lb, b->U, b->U, b->F, tape, b->lambda, b->rho);
                 };
                 std::vector<lt::LinkMap> cands;
                 for (const lt::LinkMap& c : lt::tile_candidates(g, cur.nthreads))

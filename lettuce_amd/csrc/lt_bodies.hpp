@@ -242,7 +242,10 @@ __device__ __forceinline__ int dn_rt(const Geom& g, int s, int cm, int d)
 
 // Parking slot: the first of the two products of a half waits in LDS (9 entries,
 // stride n) while the second is formed, so it holds no registers meanwhile; the empty
-// asm statements with memory clobbers keep the compiler from moving it back.
+// asm statements with memory clobbers keep the compiler from moving it back.  park is
+// never null in the kernels: testing it anyway puts the parking in a branch of its own,
+// and the compiler does not hoist the second product's link loads above a branch (19
+// spilled VGPRs in clover_deriv_all_kernel without the tests, 2 with them).
 __device__ __forceinline__ void park_store(double2* park, int n, const M3& a)
 {
 #pragma unroll
@@ -274,12 +277,12 @@ __device__ __forceinline__ M3 cderiv_upper(const double2* __restrict__ U, const 
     M3 S = mul_dn(Unu_x, R_at(C, g, s, par, RHO, SIGMA));
     S = add(S, mul_nd(R_at(C, g, snu, q, RHO, SIGMA), Unu_x));
     M3 acc = mul(tmp1, S);
-    park_store(park, n, acc);
+    if (park) park_store(park, n, acc);
     const M3 Unu_mu = ldu(U, g, smu, NU, q);
     S = mul(Unu_mu, R_at(C, g, up_rt(g, smu, pick4(c, NU), NU), par, RHO, SIGMA));
     S = add(S, mul(R_at(C, g, smu, q, RHO, SIGMA), Unu_mu));
     const M3 second = mul(S, tmp2);
-    acc = park_load(park, n);
+    if (park) acc = park_load(park, n);
     return add(acc, second);
 }
 
@@ -301,11 +304,11 @@ __device__ __forceinline__ M3 cderiv_lower(const double2* __restrict__ U, const 
     M3 S = mul(Unu_nd, R_at(C, g, s, par, RHO, SIGMA));
     S = add(S, mul(R_at(C, g, snd, q, RHO, SIGMA), Unu_nd));
     M3 acc = mul(tmp3, S);
-    park_store(park, n, acc);
+    if (park) park_store(park, n, acc);
     S = mul_dn(Unu_mund, R_at(C, g, smu_nd, par, RHO, SIGMA));
     S = add(S, mul_nd(R_at(C, g, smu, q, RHO, SIGMA), Unu_mund));
     const M3 second = mul(S, tmp4);
-    acc = park_load(park, n);
+    if (park) acc = park_load(park, n);
     return add(acc, second);
 }
 
@@ -384,54 +387,84 @@ __device__ __forceinline__ void body_lambda(const double2* __restrict__ U, const
     st_her8(Lam, g, s, mu, par, tl_herm(Gam));
 }
 
-// Sigma_new = Algebra(U (Sigma' exp(iQ) + i rho st^+ Lambda - i rho sum_nu [six-term staple of Lambda]))
-// (StoutForceRecursion, stout_smearing.hpp:177-229) in two launches: the six-term
-// sum fs into a scratch field, then the rest.  Together they need more registers
-// than one thread has without spilling.
-__device__ __forceinline__ void body_sigma_fs(const double2* __restrict__ U, const double2* __restrict__ Lam,
-                                              double2* __restrict__ Fs, const Geom& g, const LinkMap& m, int b)
+// body_sigma_fs and body_sigma in one pass: the six-term sum and the staple st are
+// built in the same loop over nu != mu from the same neighbour links (the staple with
+// the products of staple(), in its order), each waiting in LDS between the directions
+// (fs at lds + lane, st after it: entries e * nthreads, e < 9), then the sigma update.
+// The same operations in the same order as the two launches: the same Sigma bit for
+// bit, without the scratch field Fs and the second reading of the neighbours.
+// LDS: 18 x 16 bytes per lane of the block.
+__device__ __forceinline__ void body_sigma_all(const double2* __restrict__ U, double2* __restrict__ Sig,
+                                               const double2* __restrict__ T, const double2* __restrict__ Lam,
+                                               const Geom& g, const LinkMap& m, int b, double rho, double2* lds)
 {
     int mu, par, c[4], s;
     if (!map_link_b(g, m, b, mu, par, c, s)) return;
+    const int n = m.nthreads;
+    double2* fs_slot = lds + threadIdx.x;
+    double2* st_slot = fs_slot + 9 * n;
     const int q = par ^ 1;
-    const int smu = up(g, s, c[mu], mu);
-    M3 fs = m3_zero();
-#pragma unroll
-    for (int nu = 0; nu < 4; ++nu) {
-        if (nu == mu) continue;
-        const int snu = up(g, s, c[nu], nu), snd = dn(g, s, c[nu], nu);
-        const int smu_nd = dn(g, smu, c[nu], nu);
+    const int smu = up_rt(g, s, pick4(c, mu), mu);
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {
+        const int nu = k < mu ? k : k + 1;
+        const int cnu = pick4(c, nu);
+        const int snu = up_rt(g, s, cnu, nu), snd = dn_rt(g, s, cnu, nu);
+        const int smu_nd = dn_rt(g, smu, cnu, nu);
         const M3 Unu_x = ldu(U, g, s, nu, par);
         const M3 Unu_mu = ldu(U, g, smu, nu, q);
-        const M3 tmp1 = mul_nd(Unu_mu, ldu(U, g, snu, mu, q));
-        // rows 1 and 6: tmp1 (U_nu(x)^+ L_nu(x) + L_mu(x+nu) U_nu(x)^+)
-        acc(fs, mul(tmp1, add(mul_dn(Unu_x, ld_her8(Lam, g, s, nu, par)), mul_nd(ld_her8(Lam, g, snu, mu, q), Unu_x))));
-        // row 5: - L_nu(x+mu) tmp1 U_nu(x)^+
-        fs = sub(fs, mul_nd(mul(ld_her8(Lam, g, smu, nu, q), tmp1), Unu_x));
-        // rows 2, 4, 3: U_nu(x+mu-nu)^+ (U_mu(x-nu)^+ (L_mu(x-nu) - L_nu(x-nu)) + L_nu(x+mu-nu) U_mu(x-nu)^+) U_nu(x-nu)
+        M3 tmp1;
+        {
+            const M3 Umu_nu = ldu(U, g, snu, mu, q);
+            // staple, upper term: U_nu(x) U_mu(x+nu) U_nu(x+mu)^+
+            const M3 t = mul_nd_r01(mul_r01(Unu_x, Umu_nu), Unu_mu);
+            M3 st;
+            if (k == 0) {
+                st = t;
+                third_row(st);
+            } else {
+                st = park_load(st_slot, n);
+                acc_su3_r01(st, t);
+            }
+            park_store(st_slot, n, st);
+            tmp1 = mul_nd(Unu_mu, Umu_nu);
+        }
+        {   // six-term sum: + tmp1 (U_nu(x)^+ L_nu(x) + L_mu(x+nu) U_nu(x)^+) - L_nu(x+mu) tmp1 U_nu(x)^+
+            const M3 x1 = mul(tmp1, add(mul_dn(Unu_x, ld_her8(Lam, g, s, nu, par)), mul_nd(ld_her8(Lam, g, snu, mu, q), Unu_x)));
+            M3 fs = k == 0 ? m3_zero() : park_load(fs_slot, n);
+            acc(fs, x1);
+            park_store(fs_slot, n, fs);
+            const M3 x2 = mul_nd(mul(ld_her8(Lam, g, smu, nu, q), tmp1), Unu_x);
+            fs = park_load(fs_slot, n);
+            fs = sub(fs, x2);
+            park_store(fs_slot, n, fs);
+        }
         const M3 Umu_nd = ldu(U, g, snd, mu, q);
-        M3 tmp2 = mul_dn(Umu_nd, sub(ld_her8(Lam, g, snd, mu, q), ld_her8(Lam, g, snd, nu, q)));
-        M3 inner = add(tmp2, mul_nd(ld_her8(Lam, g, smu_nd, nu, par), Umu_nd));
-        acc(fs, mul(mul_dn(ldu(U, g, smu_nd, nu, par), inner), ldu(U, g, snd, nu, q)));
+        const M3 Unu_mund = ldu(U, g, smu_nd, nu, par);
+        const M3 Unu_nd = ldu(U, g, snd, nu, q);
+        {   // staple, lower term: U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu)
+            M3 ef = mul_r01(Umu_nd, Unu_mund);
+            third_row(ef);
+            M3 st = park_load(st_slot, n);
+            mul_dn_acc(st, Unu_nd, ef);
+            park_store(st_slot, n, st);
+        }
+        {   // + U_nu(x+mu-nu)^+ (U_mu(x-nu)^+ (L_mu(x-nu) - L_nu(x-nu)) + L_nu(x+mu-nu) U_mu(x-nu)^+) U_nu(x-nu)
+            M3 tmp2 = mul_dn(Umu_nd, sub(ld_her8(Lam, g, snd, mu, q), ld_her8(Lam, g, snd, nu, q)));
+            M3 inner = add(tmp2, mul_nd(ld_her8(Lam, g, smu_nd, nu, par), Umu_nd));
+            const M3 x3 = mul(mul_dn(Unu_mund, inner), Unu_nd);
+            M3 fs = park_load(fs_slot, n);
+            acc(fs, x3);
+            park_store(fs_slot, n, fs);
+        }
     }
-    st_link(Fs, g, s, mu, par, fs);
-}
-
-__device__ __forceinline__ void body_sigma(const double2* __restrict__ U, double2* __restrict__ Sig,
-                                           const double2* __restrict__ T, const double2* __restrict__ Lam,
-                                           const double2* __restrict__ Fs, const Geom& g, const LinkMap& m, int b,
-                                           double rho)
-{
-    int mu, par, c[4], s;
-    if (!map_link_b(g, m, b, mu, par, c, s)) return;
-    M3 cl = mul_dn(staple(U, g, s, c, mu), ld_her8(Lam, g, s, mu, par));   // st^+ Lambda
-    const M3 fs = ld(Fs, g, s, mu, par);
+    M3 cl = mul_dn(park_load(st_slot, n), ld_her8(Lam, g, s, mu, par));   // st^+ Lambda
+    const M3 fs = park_load(fs_slot, n);
     M3 inner = mul(ld(Sig, g, s, mu, par), su3_exp(ld_her8(T, g, s, mu, par)));
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        // + i rho (st^+ Lambda) - i rho fs
-        cd d = cl.m[k] - fs.m[k];
-        inner.m[k] += cd{-rho * d.im, rho * d.re};
+    for (int e = 0; e < 9; ++e) {
+        cd d = cl.m[e] - fs.m[e];
+        inner.m[e] += cd{-rho * d.im, rho * d.re};
     }
     st_link(Sig, g, s, mu, par, algebra(mul(ldu(U, g, s, mu, par), inner)));
 }

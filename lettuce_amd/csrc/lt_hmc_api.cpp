@@ -26,7 +26,7 @@ double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C);
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
                     double scale_factor);
 void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,
-                       double2* Sig, const double2* T, double2* Lam, double2* Fs, double rho);
+                       double2* Sig, const double2* T, double2* Lam, double rho);
 }  // namespace lt
 
 // Collective-variable workspace: stout levels with their exponent tapes, the six
@@ -52,7 +52,6 @@ struct lt_cv_s {
     const double2* input = nullptr; // checkpointed: level 0 (the field of the last forward pass)
     double2* clover = nullptr;      // 6 planes per site
     double2* lambda = nullptr;
-    double2* fsum = nullptr;        // six-term Lambda staple sums of the recursion
     size_t bytes = 0;               // device memory held
     bool forward_done = false;
     std::vector<double> path;       // CVs visited by the last MetaD-HMC trajectory
@@ -63,7 +62,7 @@ struct lt_cv_s {
         for (auto p : levels) (void)hipFree(p);
         for (auto p : tapes) (void)hipFree(p);
         for (auto p : seg) (void)hipFree(p);
-        for (void* p : {(void*)top, (void*)clover, (void*)lambda, (void*)fsum})
+        for (void* p : {(void*)top, (void*)clover, (void*)lambda})
             if (p) (void)hipFree(p);
         levels.clear();
         tapes.clear();
@@ -327,7 +326,7 @@ void cv_backward(lt_cv cv, double dvdq, double2* F)
         lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[n], cv->clover, F, dvdq);
         for (int k = n; k > 0; --k)
             lt::k_stout_recursion(cv->ctx, cv->g, cv->map, cv->levels[k - 1], cv->levels[k], F, cv->tapes[k - 1],
-                                  cv->lambda, cv->fsum, cv->rho);
+                                  cv->lambda, cv->rho);
         return;
     }
     lt::k_clover_deriv(cv->ctx, cv->g, cv->map, n ? cv->top : cv->input, cv->clover, F, dvdq);
@@ -340,7 +339,7 @@ void cv_backward(lt_cv cv, double dvdq, double2* F)
             lt::k_stout_tape(cv->ctx, cv->g, cv->map, level(j - 1), cv->seg[j - a], cv->tapes[j - a - 1], cv->rho);
         for (int j = b; j > a; --j)
             lt::k_stout_recursion(cv->ctx, cv->g, cv->map, level(j - 1), level(j), F, cv->tapes[j - a - 1],
-                                  cv->lambda, cv->fsum, cv->rho);
+                                  cv->lambda, cv->rho);
     }
 }
 
@@ -523,7 +522,6 @@ int lt_cv_create_checkpointed(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_
             }
             cv->clover = take((size_t)12 * lt::R_SLOTS * cv->g.Vh);
             cv->lambda = take(na);
-            cv->fsum = take(nl);
         } catch (...) {
             cv->release();
             delete cv;
