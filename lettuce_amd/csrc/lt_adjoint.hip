@@ -61,12 +61,13 @@ __global__ void LT_LINK_KERNEL lambda_kernel(const double2* __restrict__ U, cons
     body_lambda(U, Uprev, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x));
 }
 
+template <bool KICK>
 __global__ void LT_LINK_KERNEL sigma_all_kernel(const double2* __restrict__ U, double2* __restrict__ Sig,
                                                 const double2* __restrict__ T, const double2* __restrict__ Lam, Geom g,
-                                                LinkMap m, double rho)
+                                                LinkMap m, double rho, double2* __restrict__ P, double a, double bf)
 {
     extern __shared__ double2 lds[];
-    body_sigma_all(U, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x), rho, lds);
+    body_sigma_all<KICK>(U, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x), rho, lds, P, a, bf);
 }
 
 }  // namespace
@@ -98,12 +99,17 @@ void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* 
 }
 
 void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,
-                       double2* Sig, const double2* T, double2* Lam, double rho)
+                       double2* Sig, const double2* T, double2* Lam, double rho, const MomentumKick* kick)
 {
     const dim3 grid(8 * map_blocks(m)), block(m.nthreads);
+    const size_t lds = 18 * m.nthreads * sizeof(double2);
     launch(ctx, "stout_lambda", lambda_kernel, grid, block, 0, U, Uprev, Sig, T, Lam, g, m);
-    launch(ctx, "stout_sigma", sigma_all_kernel, grid, block, 18 * m.nthreads * sizeof(double2), U, Sig, T,
-           (const double2*)Lam, g, m, rho);
+    if (kick)
+        launch(ctx, "stout_sigma", sigma_all_kernel<true>, grid, block, lds, U, Sig, T, (const double2*)Lam, g, m, rho,
+               kick->P, kick->a, kick->bf);
+    else
+        launch(ctx, "stout_sigma", sigma_all_kernel<false>, grid, block, lds, U, Sig, T, (const double2*)Lam, g, m, rho,
+               (double2*)nullptr, 0.0, 0.0);
 }
 
 }  // namespace lt

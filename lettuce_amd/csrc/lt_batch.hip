@@ -88,13 +88,15 @@ __global__ void LT_LINK_KERNEL bk_lambda(const double2* __restrict__ U, const do
     body_lambda(U + off, Uprev + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), g, m, lb);
 }
 
+template <bool KICK>
 __global__ void LT_LINK_KERNEL bk_sigma_all(const double2* __restrict__ U, double2* __restrict__ Sig,
                                             const double2* __restrict__ T, const double2* __restrict__ Lam, BatchGrid bg,
-                                            double rho)
+                                            double rho, double2* __restrict__ P, double a, double bf)
 {
     extern __shared__ double2 lds[];
     LT_BATCH_LINK_PROLOGUE
-    body_sigma_all(U + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), g, m, lb, rho, lds);
+    body_sigma_all<KICK>(U + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), g, m, lb, rho, lds,
+                         KICK ? P + off : nullptr, a, bf);
 }
 
 #undef LT_BATCH_LINK_PROLOGUE
@@ -392,12 +394,17 @@ void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double
 }
 
 void kb_stout_recursion(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* Uprev,
-                        double2* Sig, const double2* T, double2* Lam, double rho)
+                        double2* Sig, const double2* T, double2* Lam, double rho, const MomentumKick* kick)
 {
     const dim3 grid(total_lb), block(link_threads(bg, total_lb));
+    const size_t lds = 18 * block.x * sizeof(double2);
     launch(ctx, "batch_stout_lambda", bk_lambda, grid, block, 0, U, Uprev, Sig, T, Lam, bg);
-    launch(ctx, "batch_stout_sigma", bk_sigma_all, grid, block, 18 * block.x * sizeof(double2), U, Sig, T,
-           (const double2*)Lam, bg, rho);
+    if (kick)
+        launch(ctx, "batch_stout_sigma", bk_sigma_all<true>, grid, block, lds, U, Sig, T, (const double2*)Lam, bg, rho,
+               kick->P, kick->a, kick->bf);
+    else
+        launch(ctx, "batch_stout_sigma", bk_sigma_all<false>, grid, block, lds, U, Sig, T, (const double2*)Lam, bg, rho,
+               (double2*)nullptr, 0.0, 0.0);
 }
 
 void kb_plaquette(lt_ctx ctx, const BatchGrid& bg, int total_sb, const double2* U, double* part, double* out)

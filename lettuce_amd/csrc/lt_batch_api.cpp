@@ -167,9 +167,13 @@ void enqueue_step(lt_batch b, const StepKey& p, int n_sub)
                 forward(b, bg);
                 lt::kb_dvdq(ctx, bg, st, bias, sub ? isub++ : -1);
                 lt::kb_clover_deriv(ctx, bg, b->total_lb, b->levels[b->n_levels], b->clover, b->F, st.dvdq);
+                // KICK_BOTH: UpdateMomenta applied by the last kernel of the force (level 0 = U)
+                const lt::MomentumKick mk{b->P, e * p.beta / 6.0, e};
+                const bool fused = kick == lt::KICK_BOTH && b->n_levels > 0;
                 for (int k = b->n_levels; k > 0; --k)
                     lt::kb_stout_recursion(ctx, bg, b->total_lb, b->levels[k - 1], b->levels[k], b->F,
-                                           b->tapes[k - 1], b->lambda, b->rho);
+                                           b->tapes[k - 1], b->lambda, b->rho, fused && k == 1 ? &mk : nullptr);
+                if (fused) return;
             }
             if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)
                 lt::kb_force_update(ctx, bg, b->total_lb, b->U, b->P, b->F, e * p.beta / 6.0, e);

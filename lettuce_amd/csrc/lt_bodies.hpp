@@ -394,9 +394,18 @@ __device__ __forceinline__ void body_lambda(const double2* __restrict__ U, const
 // The same operations in the same order as the two launches: the same Sigma bit for
 // bit, without the scratch field Fs and the second reading of the neighbours.
 // LDS: 18 x 16 bytes per lane of the block.
+// KICK (the last level of the recursion inside a metadynamics momentum update): the
+// result is the bias force at level 0, and instead of storing it the kernel applies
+// UpdateMomenta to the link, P += a Algebra(st U^+) + bf * force, with the staple it
+// already holds -- body_force_update's formula, without the force field's store and
+// load and the staple formed again (the momenta agree with the separate update to
+// rounding, not bit for bit: the products are contracted into fused multiply-adds
+// differently).
+template <bool KICK>
 __device__ __forceinline__ void body_sigma_all(const double2* __restrict__ U, double2* __restrict__ Sig,
                                                const double2* __restrict__ T, const double2* __restrict__ Lam,
-                                               const Geom& g, const LinkMap& m, int b, double rho, double2* lds)
+                                               const Geom& g, const LinkMap& m, int b, double rho, double2* lds,
+                                               double2* __restrict__ P = nullptr, double a = 0.0, double bf = 0.0)
 {
     int mu, par, c[4], s;
     if (!map_link_b(g, m, b, mu, par, c, s)) return;
@@ -466,7 +475,17 @@ __device__ __forceinline__ void body_sigma_all(const double2* __restrict__ U, do
         cd d = cl.m[e] - fs.m[e];
         inner.m[e] += cd{-rho * d.im, rho * d.re};
     }
-    st_link(Sig, g, s, mu, par, algebra(mul(ldu(U, g, s, mu, par), inner)));
+    const M3 L = ldu(U, g, s, mu, par);
+    const M3 sig = algebra(mul(L, inner));
+    if (!KICK) {
+        st_link(Sig, g, s, mu, par, sig);
+        return;
+    }
+    const M3 A = algebra(mul_nd(park_load(st_slot, n), L));
+    M3 p = ld(P, g, s, mu, par);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) p.m[e] += a * A.m[e] + bf * sig.m[e];
+    st_link(P, g, s, mu, par, p);
 }
 
 }  // namespace lt

@@ -25,8 +25,9 @@ void k_stout_tape(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in
 double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C);
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
                     double scale_factor);
+// kick: applied by the level's last kernel instead of storing Sig (the last level only)
 void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,
-                       double2* Sig, const double2* T, double2* Lam, double rho);
+                       double2* Sig, const double2* T, double2* Lam, double rho, const MomentumKick* kick = nullptr);
 }  // namespace lt
 
 // Collective-variable workspace: stout levels with their exponent tapes, the six
@@ -318,15 +319,17 @@ double cv_forward(lt_cv cv, lt_field U)
     return Q;
 }
 
-// CV adjoint into F: dvdq * dQ/dU through the clover derivative and n recursions
-void cv_backward(lt_cv cv, double dvdq, double2* F)
+// CV adjoint into F: dvdq * dQ/dU through the clover derivative and n recursions.
+// With kick (n > 0 only) the last recursion step applies the momentum update to
+// kick->P instead of storing the level-0 force, and F is left holding level 1's.
+void cv_backward(lt_cv cv, double dvdq, double2* F, const lt::MomentumKick* kick = nullptr)
 {
     const int n = cv->n_levels;
     if (cv->interval == 0) {
         lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[n], cv->clover, F, dvdq);
         for (int k = n; k > 0; --k)
             lt::k_stout_recursion(cv->ctx, cv->g, cv->map, cv->levels[k - 1], cv->levels[k], F, cv->tapes[k - 1],
-                                  cv->lambda, cv->rho);
+                                  cv->lambda, cv->rho, k == 1 ? kick : nullptr);
         return;
     }
     lt::k_clover_deriv(cv->ctx, cv->g, cv->map, n ? cv->top : cv->input, cv->clover, F, dvdq);
@@ -339,7 +342,7 @@ void cv_backward(lt_cv cv, double dvdq, double2* F)
             lt::k_stout_tape(cv->ctx, cv->g, cv->map, level(j - 1), cv->seg[j - a], cv->tapes[j - a - 1], cv->rho);
         for (int j = b; j > a; --j)
             lt::k_stout_recursion(cv->ctx, cv->g, cv->map, level(j - 1), level(j), F, cv->tapes[j - a - 1],
-                                  cv->lambda, cv->rho);
+                                  cv->lambda, cv->rho, j == 1 ? kick : nullptr);
     }
 }
 
@@ -361,6 +364,12 @@ void metad_integrate(lt_field U, lt_field P, lt_field F, lt_cv cv, lt_bias bias,
                 // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436)
                 const double q = cv_forward(cv, U);
                 if (sub) samples.push_back(q);
+                if (kick == lt::KICK_BOTH && cv->n_levels > 0) {
+                    // UpdateMomenta applied by the last kernel of the force (on level 0 = U)
+                    const lt::MomentumKick mk{P->d, e * beta / 6.0, e};
+                    cv_backward(cv, bias->derivative(q), F->d, &mk);
+                    return;
+                }
                 cv_backward(cv, bias->derivative(q), F->d);
             }
             if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)

@@ -204,6 +204,12 @@ inline bool valid_integrator(int k)
 
 // Momentum kicks of a MetaD-HMC schedule: gauge force, bias force, or both in one update.
 enum Kick { KICK_GAUGE = 1, KICK_BIAS = 2, KICK_BOTH = 3 };
+// A KICK_BOTH momentum update P += a Algebra(st U^+) + bf F applied by the last kernel
+// of the force computation instead of a separate pass over the stored force F.
+struct MomentumKick {
+    double2* P;
+    double a, bf;
+};
 // The MetaD-HMC sequence: the single-scale integrators kick with both forces
 // (HMCMetaDKernel::UpdateMomenta, hmc_metadynamics.hpp:441-459); the multiple-timescale
 // ones (hmc_metadynamics.hpp:28-238) kick the bias force on the outer scale and the
