@@ -61,13 +61,13 @@ __global__ void LT_LINK_KERNEL lambda_kernel(const double2* __restrict__ U, cons
     body_lambda(U, Uprev, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x));
 }
 
-template <bool KICK>
+template <int MODE>
 __global__ void LT_LINK_KERNEL sigma_all_kernel(const double2* __restrict__ U, double2* __restrict__ Sig,
                                                 const double2* __restrict__ T, const double2* __restrict__ Lam, Geom g,
-                                                LinkMap m, double rho, double2* __restrict__ P, double a, double bf)
+                                                LinkMap m, double rho, MomentumKick kick, SigmaNext next)
 {
     extern __shared__ double2 lds[];
-    body_sigma_all<KICK>(U, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x), rho, lds, P, a, bf);
+    body_sigma_all<MODE>(U, Sig, T, Lam, g, m, link_block(m, blockIdx.x, gridDim.x), rho, lds, kick, next);
 }
 
 }  // namespace
@@ -98,18 +98,31 @@ void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* 
            scale_factor);
 }
 
-void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,
-                       double2* Sig, const double2* T, double2* Lam, double rho, const MomentumKick* kick)
+void k_stout_recursion_run(lt_ctx ctx, const Geom& g, const LinkMap& m, const LevelFn& level, const LevelFn& tape,
+                           int hi, int lo, double2* Sig, double2* const lam[2], double rho, const MomentumKick* kick)
 {
     const dim3 grid(8 * map_blocks(m)), block(m.nthreads);
     const size_t lds = 18 * m.nthreads * sizeof(double2);
-    launch(ctx, "stout_lambda", lambda_kernel, grid, block, 0, U, Uprev, Sig, T, Lam, g, m);
-    if (kick)
-        launch(ctx, "stout_sigma", sigma_all_kernel<true>, grid, block, lds, U, Sig, T, (const double2*)Lam, g, m, rho,
-               kick->P, kick->a, kick->bf);
-    else
-        launch(ctx, "stout_sigma", sigma_all_kernel<false>, grid, block, lds, U, Sig, T, (const double2*)Lam, g, m, rho,
-               (double2*)nullptr, 0.0, 0.0);
+    int cur = 0;
+    launch(ctx, "stout_lambda", lambda_kernel, grid, block, 0, level(hi - 1), level(hi), Sig, tape(hi - 1), lam[cur], g,
+           m);
+    for (int k = hi; k > lo; --k) {
+        const double2* U = level(k - 1);
+        const double2* T = tape(k - 1);
+        const double2* L = lam[cur];
+        if (k - 1 > lo) {
+            const SigmaNext next{level(k - 2), tape(k - 2), lam[cur ^ 1]};
+            launch(ctx, "stout_sigma", sigma_all_kernel<SIGMA_LAMBDA>, grid, block, lds, U, Sig, T, L, g, m, rho,
+                   MomentumKick{}, next);
+            cur ^= 1;
+        } else if (kick && k == 1) {
+            launch(ctx, "stout_sigma", sigma_all_kernel<SIGMA_KICK>, grid, block, lds, U, Sig, T, L, g, m, rho, *kick,
+                   SigmaNext{});
+        } else {
+            launch(ctx, "stout_sigma", sigma_all_kernel<SIGMA_STORE>, grid, block, lds, U, Sig, T, L, g, m, rho,
+                   MomentumKick{}, SigmaNext{});
+        }
+    }
 }
 
 }  // namespace lt

@@ -88,15 +88,16 @@ __global__ void LT_LINK_KERNEL bk_lambda(const double2* __restrict__ U, const do
     body_lambda(U + off, Uprev + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), g, m, lb);
 }
 
-template <bool KICK>
+template <int MODE>
 __global__ void LT_LINK_KERNEL bk_sigma_all(const double2* __restrict__ U, double2* __restrict__ Sig,
                                             const double2* __restrict__ T, const double2* __restrict__ Lam, BatchGrid bg,
-                                            double rho, double2* __restrict__ P, double a, double bf)
+                                            double rho, MomentumKick kick, SigmaNext next)
 {
     extern __shared__ double2 lds[];
     LT_BATCH_LINK_PROLOGUE
-    body_sigma_all<KICK>(U + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), g, m, lb, rho, lds,
-                         KICK ? P + off : nullptr, a, bf);
+    if (MODE == SIGMA_KICK) kick.P += off;
+    if (MODE == SIGMA_LAMBDA) next = SigmaNext{next.U + off, next.T + alg_offset(off), next.Lam + alg_offset(off)};
+    body_sigma_all<MODE>(U + off, Sig + off, T + alg_offset(off), Lam + alg_offset(off), g, m, lb, rho, lds, kick, next);
 }
 
 #undef LT_BATCH_LINK_PROLOGUE
@@ -393,18 +394,31 @@ void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double
     launch(ctx, "batch_clover_deriv", bk_clover_deriv_all, grid, block, 18 * block.x * sizeof(double2), U, C, F, bg, scale);
 }
 
-void kb_stout_recursion(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* Uprev,
-                        double2* Sig, const double2* T, double2* Lam, double rho, const MomentumKick* kick)
+void kb_stout_recursion_run(lt_ctx ctx, const BatchGrid& bg, int total_lb, const LevelFn& level, const LevelFn& tape,
+                            int hi, int lo, double2* Sig, double2* const lam[2], double rho, const MomentumKick* kick)
 {
     const dim3 grid(total_lb), block(link_threads(bg, total_lb));
     const size_t lds = 18 * block.x * sizeof(double2);
-    launch(ctx, "batch_stout_lambda", bk_lambda, grid, block, 0, U, Uprev, Sig, T, Lam, bg);
-    if (kick)
-        launch(ctx, "batch_stout_sigma", bk_sigma_all<true>, grid, block, lds, U, Sig, T, (const double2*)Lam, bg, rho,
-               kick->P, kick->a, kick->bf);
-    else
-        launch(ctx, "batch_stout_sigma", bk_sigma_all<false>, grid, block, lds, U, Sig, T, (const double2*)Lam, bg, rho,
-               (double2*)nullptr, 0.0, 0.0);
+    int cur = 0;
+    launch(ctx, "batch_stout_lambda", bk_lambda, grid, block, 0, level(hi - 1), level(hi), Sig, tape(hi - 1), lam[cur],
+           bg);
+    for (int k = hi; k > lo; --k) {
+        const double2* U = level(k - 1);
+        const double2* T = tape(k - 1);
+        const double2* L = lam[cur];
+        if (k - 1 > lo) {
+            const SigmaNext next{level(k - 2), tape(k - 2), lam[cur ^ 1]};
+            launch(ctx, "batch_stout_sigma", bk_sigma_all<SIGMA_LAMBDA>, grid, block, lds, U, Sig, T, L, bg, rho,
+                   MomentumKick{}, next);
+            cur ^= 1;
+        } else if (kick && k == 1) {
+            launch(ctx, "batch_stout_sigma", bk_sigma_all<SIGMA_KICK>, grid, block, lds, U, Sig, T, L, bg, rho, *kick,
+                   SigmaNext{});
+        } else {
+            launch(ctx, "batch_stout_sigma", bk_sigma_all<SIGMA_STORE>, grid, block, lds, U, Sig, T, L, bg, rho,
+                   MomentumKick{}, SigmaNext{});
+        }
+    }
 }
 
 void kb_plaquette(lt_ctx ctx, const BatchGrid& bg, int total_sb, const double2* U, double* part, double* out)

@@ -77,9 +77,10 @@ void kb_clover_charge(lt_ctx ctx, const BatchGrid& bg, int total_sb, const doubl
                       double* qraw);
 void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* C, double2* F,
                      const double* scale);
-// kick: applied by the level's last kernel instead of storing Sig (the last level only)
-void kb_stout_recursion(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* Uprev,
-                        double2* Sig, const double2* T, double2* Lam, double rho, const MomentumKick* kick = nullptr);
+// the stout-force recursion of levels hi..lo+1 (see k_stout_recursion_run)
+void kb_stout_recursion_run(lt_ctx ctx, const BatchGrid& bg, int total_lb, const LevelFn& level, const LevelFn& tape,
+                            int hi, int lo, double2* Sig, double2* const lam[2], double rho,
+                            const MomentumKick* kick = nullptr);
 void kb_plaquette(lt_ctx ctx, const BatchGrid& bg, int total_sb, const double2* U, double* part, double* out);
 void kb_kinetic(lt_ctx ctx, const BatchGrid& bg, int total_kb, const double2* P, double* part, double* out);
 // scalar stages (one thread per replica)

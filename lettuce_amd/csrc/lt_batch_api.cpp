@@ -56,7 +56,8 @@ struct lt_batch_s {
 
     lt::BatchRep* d_reps = nullptr;
     int* d_pre = nullptr;
-    double2 *U = nullptr, *Ucopy = nullptr, *P = nullptr, *F = nullptr, *lambda = nullptr, *clover = nullptr;
+    double2 *U = nullptr, *Ucopy = nullptr, *P = nullptr, *F = nullptr, *clover = nullptr;
+    double2* lambda[2] = {nullptr, nullptr};   // Lambda of alternate levels
     std::vector<double2*> levels;          // levels[0] aliases U
     std::vector<double2*> tapes;
     double* part = nullptr;
@@ -104,7 +105,7 @@ struct lt_batch_s {
         for (auto p : tapes) (void)hipFree(p);
         levels.clear();
         tapes.clear();
-        for (void* p : {(void*)d_reps, (void*)d_pre, (void*)U, (void*)Ucopy, (void*)P, (void*)F, (void*)lambda,
+        for (void* p : {(void*)d_reps, (void*)d_pre, (void*)U, (void*)Ucopy, (void*)P, (void*)F, (void*)lambda[0], (void*)lambda[1],
                         (void*)clover, (void*)part, (void*)state_mem, (void*)st.samples,
                         (void*)bias.grid})
             if (p) (void)hipFree(p);
@@ -170,9 +171,11 @@ void enqueue_step(lt_batch b, const StepKey& p, int n_sub)
                 // KICK_BOTH: UpdateMomenta applied by the last kernel of the force (level 0 = U)
                 const lt::MomentumKick mk{b->P, e * p.beta / 6.0, e};
                 const bool fused = kick == lt::KICK_BOTH && b->n_levels > 0;
-                for (int k = b->n_levels; k > 0; --k)
-                    lt::kb_stout_recursion(ctx, bg, b->total_lb, b->levels[k - 1], b->levels[k], b->F,
-                                           b->tapes[k - 1], b->lambda, b->rho, fused && k == 1 ? &mk : nullptr);
+                if (b->n_levels > 0)
+                    lt::kb_stout_recursion_run(
+                        ctx, bg, b->total_lb, [&](int j) -> const double2* { return b->levels[j]; },
+                        [&](int j) -> const double2* { return b->tapes[j]; }, b->n_levels, 0, b->F, b->lambda, b->rho,
+                        fused ? &mk : nullptr);
                 if (fused) return;
             }
             if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)
@@ -261,7 +264,8 @@ int lt_batch_create(lt_ctx ctx, int n_rep, const int* shapes, int n_levels, doub
             b->Ucopy = dev_alloc<double2>(nl);
             b->P = dev_alloc<double2>(nl);
             b->F = dev_alloc<double2>(nl);
-            b->lambda = dev_alloc<double2>((size_t)lt::alg_offset(b->total));
+            b->lambda[0] = dev_alloc<double2>((size_t)lt::alg_offset(b->total));
+            b->lambda[1] = dev_alloc<double2>((size_t)lt::alg_offset(b->total));
             b->clover = dev_alloc<double2>((size_t)lt::clover_offset(b->total));
             b->levels.push_back(b->U);
             for (int l = 0; l < n_levels; ++l) {
@@ -497,10 +501,12 @@ int lt_batch_autotune(lt_batch b, const char* cache_path)
                 const lt::BatchGrid bg = b->grid();
                 auto run = [&](const lt::LinkMap& m) {
                     apply(m);
-                    double2* tape = b->n_levels > 0 ? b->tapes[0] : b->lambda;
+                    double2* tape = b->n_levels > 0 ? b->tapes[0] : b->lambda[1];
                     if (b->n_levels > 0)   // never writes level 0 (the replicas' fields)
                         lt::kb_stout_tape(b->ctx, bg, b->total_lb, b->levels[0], b->levels[1], tape, b->rho);
-                    lt::kb_stout_recursion(b->ctx, bg, b->total_lb, b->U, b->U, b->F, tape, b->lambda, b->rho);
+                    lt::kb_stout_recursion_run(
+                        b->ctx, bg, b->total_lb, [&](int) -> const double2* { return b->U; },
+                        [&](int) -> const double2* { return tape; }, 1, 0, b->F, b->lambda, b->rho);
                 };
                 std::vector<lt::LinkMap> cands;
                 for (const lt::LinkMap& c : lt::tile_candidates(g, cur.nthreads))

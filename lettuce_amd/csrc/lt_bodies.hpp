@@ -363,13 +363,12 @@ __device__ __forceinline__ void body_clover_deriv_all(const double2* __restrict_
 
 // Sigma' = Uprev^+ Sigma (stored in place) and
 // Lambda = TracelessHermitian(tr(B1 G) Q + tr(B2 G) Q^2 + f1 G + f2 (Q G + G Q)), G = U Sigma'
-__device__ __forceinline__ void body_lambda(const double2* __restrict__ U, const double2* __restrict__ Uprev,
+// one link, with Uprev's link and Sigma given
+__device__ __forceinline__ void lambda_link(const double2* __restrict__ U, const M3& Lprev, const M3& sig,
                                             double2* __restrict__ Sig, const double2* __restrict__ T,
-                                            double2* __restrict__ Lam, const Geom& g, const LinkMap& m, int b)
+                                            double2* __restrict__ Lam, const Geom& g, int s, int mu, int par)
 {
-    int mu, par, c[4], s;
-    if (!map_link_b(g, m, b, mu, par, c, s)) return;
-    M3 sp = mul_dn(ldu(Uprev, g, s, mu, par), ld(Sig, g, s, mu, par));
+    M3 sp = mul_dn(Lprev, sig);
     st_link(Sig, g, s, mu, par, sp);
     M3 Q = ld_her8(T, g, s, mu, par);
     M3 Q2 = mul(Q, Q);
@@ -386,6 +385,14 @@ __device__ __forceinline__ void body_lambda(const double2* __restrict__ U, const
     for (int k = 0; k < 9; ++k) Gam.m[k] = t1 * Q.m[k] + t2 * Q2.m[k] + f1 * G.m[k] + f2 * QG.m[k];
     st_her8(Lam, g, s, mu, par, tl_herm(Gam));
 }
+__device__ __forceinline__ void body_lambda(const double2* __restrict__ U, const double2* __restrict__ Uprev,
+                                            double2* __restrict__ Sig, const double2* __restrict__ T,
+                                            double2* __restrict__ Lam, const Geom& g, const LinkMap& m, int b)
+{
+    int mu, par, c[4], s;
+    if (!map_link_b(g, m, b, mu, par, c, s)) return;
+    lambda_link(U, ldu(Uprev, g, s, mu, par), ld(Sig, g, s, mu, par), Sig, T, Lam, g, s, mu, par);
+}
 
 // body_sigma_fs and body_sigma in one pass: the six-term sum and the staple st are
 // built in the same loop over nu != mu from the same neighbour links (the staple with
@@ -394,18 +401,25 @@ __device__ __forceinline__ void body_lambda(const double2* __restrict__ U, const
 // The same operations in the same order as the two launches: the same Sigma bit for
 // bit, without the scratch field Fs and the second reading of the neighbours.
 // LDS: 18 x 16 bytes per lane of the block.
-// KICK (the last level of the recursion inside a metadynamics momentum update): the
+// MODE SIGMA_STORE stores the result in Sig.  SIGMA_LAMBDA goes on with the Lambda of the
+// level below for the same link (lambda_link: its Uprev is this level's U, whose link is
+// at hand, and Sigma the result in registers), with that level's field, tape and Lambda
+// buffer in next (not the Lambda buffer this launch reads: other lanes still read their
+// neighbours' entries there); the same operations as the two launches, without the store
+// and load of Sigma between them.
+// SIGMA_KICK (the last level of the recursion inside a metadynamics momentum update): the
 // result is the bias force at level 0, and instead of storing it the kernel applies
 // UpdateMomenta to the link, P += a Algebra(st U^+) + bf * force, with the staple it
 // already holds -- body_force_update's formula, without the force field's store and
 // load and the staple formed again (the momenta agree with the separate update to
 // rounding, not bit for bit: the products are contracted into fused multiply-adds
 // differently).
-template <bool KICK>
+enum SigmaMode { SIGMA_STORE = 0, SIGMA_KICK = 1, SIGMA_LAMBDA = 2 };
+template <int MODE>
 __device__ __forceinline__ void body_sigma_all(const double2* __restrict__ U, double2* __restrict__ Sig,
                                                const double2* __restrict__ T, const double2* __restrict__ Lam,
                                                const Geom& g, const LinkMap& m, int b, double rho, double2* lds,
-                                               double2* __restrict__ P = nullptr, double a = 0.0, double bf = 0.0)
+                                               const MomentumKick& kick, const SigmaNext& next)
 {
     int mu, par, c[4], s;
     if (!map_link_b(g, m, b, mu, par, c, s)) return;
@@ -477,15 +491,17 @@ __device__ __forceinline__ void body_sigma_all(const double2* __restrict__ U, do
     }
     const M3 L = ldu(U, g, s, mu, par);
     const M3 sig = algebra(mul(L, inner));
-    if (!KICK) {
+    if (MODE == SIGMA_STORE) {
         st_link(Sig, g, s, mu, par, sig);
-        return;
-    }
-    const M3 A = algebra(mul_nd(park_load(st_slot, n), L));
-    M3 p = ld(P, g, s, mu, par);
+    } else if (MODE == SIGMA_LAMBDA) {
+        lambda_link(next.U, L, sig, Sig, next.T, next.Lam, g, s, mu, par);
+    } else {
+        const M3 A = algebra(mul_nd(park_load(st_slot, n), L));
+        M3 p = ld(kick.P, g, s, mu, par);
 #pragma unroll
-    for (int e = 0; e < 9; ++e) p.m[e] += a * A.m[e] + bf * sig.m[e];
-    st_link(P, g, s, mu, par, p);
+        for (int e = 0; e < 9; ++e) p.m[e] += kick.a * A.m[e] + kick.bf * sig.m[e];
+        st_link(kick.P, g, s, mu, par, p);
+    }
 }
 
 }  // namespace lt

@@ -25,9 +25,7 @@ void k_stout_tape(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in
 double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C);
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
                     double scale_factor);
-// kick: applied by the level's last kernel instead of storing Sig (the last level only)
-void k_stout_recursion(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* Uprev,
-                       double2* Sig, const double2* T, double2* Lam, double rho, const MomentumKick* kick = nullptr);
+
 }  // namespace lt
 
 // Collective-variable workspace: stout levels with their exponent tapes, the six
@@ -52,7 +50,7 @@ struct lt_cv_s {
     double2* top = nullptr;         // checkpointed: level n
     const double2* input = nullptr; // checkpointed: level 0 (the field of the last forward pass)
     double2* clover = nullptr;      // 6 planes per site
-    double2* lambda = nullptr;
+    double2* lambda[2] = {nullptr, nullptr};   // Lambda of alternate levels
     size_t bytes = 0;               // device memory held
     bool forward_done = false;
     std::vector<double> path;       // CVs visited by the last MetaD-HMC trajectory
@@ -63,7 +61,7 @@ struct lt_cv_s {
         for (auto p : levels) (void)hipFree(p);
         for (auto p : tapes) (void)hipFree(p);
         for (auto p : seg) (void)hipFree(p);
-        for (void* p : {(void*)top, (void*)clover, (void*)lambda})
+        for (void* p : {(void*)top, (void*)clover, (void*)lambda[0], (void*)lambda[1]})
             if (p) (void)hipFree(p);
         levels.clear();
         tapes.clear();
@@ -327,9 +325,10 @@ void cv_backward(lt_cv cv, double dvdq, double2* F, const lt::MomentumKick* kick
     const int n = cv->n_levels;
     if (cv->interval == 0) {
         lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[n], cv->clover, F, dvdq);
-        for (int k = n; k > 0; --k)
-            lt::k_stout_recursion(cv->ctx, cv->g, cv->map, cv->levels[k - 1], cv->levels[k], F, cv->tapes[k - 1],
-                                  cv->lambda, cv->rho, k == 1 ? kick : nullptr);
+        if (n > 0)
+            lt::k_stout_recursion_run(
+                cv->ctx, cv->g, cv->map, [&](int j) -> const double2* { return cv->levels[j]; },
+                [&](int j) -> const double2* { return cv->tapes[j]; }, n, 0, F, cv->lambda, cv->rho, kick);
         return;
     }
     lt::k_clover_deriv(cv->ctx, cv->g, cv->map, n ? cv->top : cv->input, cv->clover, F, dvdq);
@@ -340,9 +339,9 @@ void cv_backward(lt_cv cv, double dvdq, double2* F, const lt::MomentumKick* kick
         auto level = [&](int j) -> const double2* { return j == a ? ckpt_level(cv, a) : cv->seg[j - a]; };
         for (int j = a + 1; j <= b; ++j)
             lt::k_stout_tape(cv->ctx, cv->g, cv->map, level(j - 1), cv->seg[j - a], cv->tapes[j - a - 1], cv->rho);
-        for (int j = b; j > a; --j)
-            lt::k_stout_recursion(cv->ctx, cv->g, cv->map, level(j - 1), level(j), F, cv->tapes[j - a - 1],
-                                  cv->lambda, cv->rho, j == 1 ? kick : nullptr);
+        lt::k_stout_recursion_run(
+            cv->ctx, cv->g, cv->map, level, [&](int j) -> const double2* { return cv->tapes[j - a]; }, b, a, F,
+            cv->lambda, cv->rho, a == 0 ? kick : nullptr);
     }
 }
 
@@ -530,7 +529,8 @@ int lt_cv_create_checkpointed(lt_ctx ctx, int nt, int nx, int ny, int nz, int n_
                 for (int j = 0; j < k; ++j) cv->tapes.push_back(take(na));
             }
             cv->clover = take((size_t)12 * lt::R_SLOTS * cv->g.Vh);
-            cv->lambda = take(na);
+            cv->lambda[0] = take(na);
+            cv->lambda[1] = take(na);
         } catch (...) {
             cv->release();
             delete cv;

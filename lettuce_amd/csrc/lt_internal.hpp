@@ -210,6 +210,24 @@ struct MomentumKick {
     double2* P;
     double a, bf;
 };
+// Fields of a stout stack by level: level(j) is level j's field, tape(j) the exponent
+// tape of the step from level j to level j + 1.
+typedef std::function<const double2*(int)> LevelFn;
+// The stout-force recursion (StoutForceRecursion, stout_smearing.hpp:122-234) from level
+// hi down to level lo: the Lambda of level hi, then for k = hi..lo+1 the sigma update of
+// level k, fused with the Lambda of level k-1 while k-1 > lo (the two Lambda buffers
+// alternate); Sig holds the force, propagated in place.  kick: applied by the update of
+// level 1 instead of storing the level-0 force (lo == 0 only).
+void k_stout_recursion_run(lt_ctx ctx, const Geom& g, const LinkMap& m, const LevelFn& level, const LevelFn& tape,
+                           int hi, int lo, double2* Sig, double2* const lam[2], double rho,
+                           const MomentumKick* kick = nullptr);
+// The level below in a stout-force recursion step fused with its Lambda: its input
+// field, its exponent tape and the Lambda buffer to write.
+struct SigmaNext {
+    const double2* U;
+    const double2* T;
+    double2* Lam;
+};
 // The MetaD-HMC sequence: the single-scale integrators kick with both forces
 // (HMCMetaDKernel::UpdateMomenta, hmc_metadynamics.hpp:441-459); the multiple-timescale
 // ones (hmc_metadynamics.hpp:28-238) kick the bias force on the outer scale and the
