@@ -15,12 +15,15 @@ Roofline: the dominant kernel is the overrelaxation colour kernel (4/5 of the
 sweeps).  Algorithmic bytes per link update = 1296 B (read the 6 staple links of
 the other 3 directions over both parities, the opposite-parity half of the own
 direction, read + write the own half: 36 * V * 144 B per sweep over 4V link
-updates); one launch updates V/2 links.  Launch durations come from HIP events
-recorded around every launch on the library's stream.
+updates); one launch updates V/2 links.  The average launch duration comes from two HIP
+events on the library's stream around 4 overrelaxation sweeps (32 launches back to back);
+the per-launch event pairs of the library's profiler (which add the events' own cost) are
+reported beside it.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size L] [--therm N] [--no-pt] [--no-48]
 """
 import argparse
+import ctypes
 import json
 import os
 import subprocess
@@ -36,6 +39,32 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_LINK_UPDATE = 1296.0
 BYTES_PER_LINK_STAGE = 480.0
 LAYOUT_BYTES_PER_LINK_UPDATE = 864.0   # 8 half-field reads + 1 write of 96-byte (two-row) links
+
+
+def block_ms(stream, fn, reps):
+    """Milliseconds per call of fn, between two HIP events recorded on the library's stream
+    around reps back-to-back calls (HIP runtime C API; one warm-up call first)."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for e in ev:
+        if hip.hipEventCreate(ctypes.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+    try:
+        s = ctypes.c_void_p(stream)
+        fn()
+        if hip.hipEventRecord(ev[0], s) != 0:
+            raise RuntimeError("hipEventRecord failed")
+        for _ in range(reps):
+            fn()
+        if hip.hipEventRecord(ev[1], s) != 0 or hip.hipEventSynchronize(ev[1]) != 0:
+            raise RuntimeError("hipEventRecord / hipEventSynchronize failed")
+        ms = ctypes.c_float()
+        if hip.hipEventElapsedTime(ctypes.byref(ms), ev[0], ev[1]) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return ms.value / reps
+    finally:
+        for e in ev:
+            hip.hipEventDestroy(e)
 
 
 def measured_traffic(path):
@@ -404,8 +433,12 @@ def main():
     n_or, ms_or = ctx.kernel_time("or_cb2")
     n_hb, ms_hb = ctx.kernel_time("hb_cb2")
     ctx.profile(False)
-    or_ms = ms_or / max(n_or, 1)
-    hb_ms = ms_hb / max(n_hb, 1)
+    or_ms_events = ms_or / max(n_or, 1)   # an event pair around every launch (adds the events' cost)
+    hb_ms_events = ms_hb / max(n_hb, 1)
+    # the colour launches back to back, one event pair around each block of sweeps on the
+    # library's stream: 4 OR sweeps = 32 launches, 1 HB sweep = 8 launches
+    or_ms = block_ms(ctx.stream(), lambda: la.Iterator.Checkerboard(orx, 4), 5) / 32
+    hb_ms = block_ms(ctx.stream(), lambda: la.Iterator.Checkerboard(hb, 1), 10) / 8
     bytes_per_launch = BYTES_PER_LINK_UPDATE * V / 2
     or_gbs = bytes_per_launch / (or_ms * 1e-3) / 1e9
     hb_gbs = bytes_per_launch / (hb_ms * 1e-3) / 1e9
@@ -511,10 +544,11 @@ def main():
                      "frac": or_gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic(os.path.join(ROOT, "profiles", f"or_cb2_{L}x4_traffic.json")),
                      "kernel": "or_cb2 (one colour of one direction)", "avg_launch_ms": or_ms,
+                     "avg_launch_ms_event_pair_per_launch": or_ms_events,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      # links are stored as two rows (96 B): the minimum this layout moves
                      "layout_bytes_per_launch": LAYOUT_BYTES_PER_LINK_UPDATE * V / 2},
-        "hb_kernel": {"avg_launch_ms": hb_ms, "achieved_GBs": hb_gbs},
+        "hb_kernel": {"avg_launch_ms": hb_ms, "achieved_GBs": hb_gbs, "avg_launch_ms_event_pair_per_launch": hb_ms_events},
         "hbm_GBs_per_update": BYTES_PER_LINK_UPDATE * 5 * 4 * V / (ms_per_step * 1e-3) / 1e9,
         "flow_rk3": {"ms_per_step": flow_ms, "model_GBs": flow_gbs, "stage_avg_launch_ms": stage_ms,
                      "stage_GBs": BYTES_PER_LINK_STAGE * 4 * V / (stage_ms * 1e-3) / 1e9,
