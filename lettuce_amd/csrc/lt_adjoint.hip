@@ -47,11 +47,13 @@ __global__ void __launch_bounds__(CBS) sum_kernel(const double* part, int n, dou
     block_sum_to<CBS>(a, out);
 }
 
+template <bool LAMBDA>
 __global__ void LT_LINK_KERNEL clover_deriv_all_kernel(const double2* __restrict__ U, const double2* __restrict__ C,
-                                                       double2* __restrict__ F, Geom g, LinkMap m, double scale_factor)
+                                                       double2* __restrict__ F, Geom g, LinkMap m, double scale_factor,
+                                                       SigmaNext next)
 {
     extern __shared__ double2 acc[];
-    body_clover_deriv_all(U, C, F, g, m, link_block(m, blockIdx.x, gridDim.x), scale_factor, acc);
+    body_clover_deriv_all<LAMBDA>(U, C, F, g, m, link_block(m, blockIdx.x, gridDim.x), scale_factor, acc, next);
 }
 
 __global__ void LT_LINK_KERNEL lambda_kernel(const double2* __restrict__ U, const double2* __restrict__ Uprev,
@@ -91,21 +93,27 @@ double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C)
 }
 
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
-                    double scale_factor)
+                    double scale_factor, const SigmaNext* next)
 {
     const dim3 grid(8 * map_blocks(m)), block(m.nthreads);
-    launch(ctx, "clover_deriv", clover_deriv_all_kernel, grid, block, 18 * m.nthreads * sizeof(double2), U, C, F, g, m,
-           scale_factor);
+    const size_t lds = 18 * m.nthreads * sizeof(double2);
+    if (next)
+        launch(ctx, "clover_deriv", clover_deriv_all_kernel<true>, grid, block, lds, U, C, F, g, m, scale_factor, *next);
+    else
+        launch(ctx, "clover_deriv", clover_deriv_all_kernel<false>, grid, block, lds, U, C, F, g, m, scale_factor,
+               SigmaNext{});
 }
 
 void k_stout_recursion_run(lt_ctx ctx, const Geom& g, const LinkMap& m, const LevelFn& level, const LevelFn& tape,
-                           int hi, int lo, double2* Sig, double2* const lam[2], double rho, const MomentumKick* kick)
+                           int hi, int lo, double2* Sig, double2* const lam[2], double rho, const MomentumKick* kick,
+                           bool top_lambda_done)
 {
     const dim3 grid(8 * map_blocks(m)), block(m.nthreads);
     const size_t lds = 18 * m.nthreads * sizeof(double2);
     int cur = 0;
-    launch(ctx, "stout_lambda", lambda_kernel, grid, block, 0, level(hi - 1), level(hi), Sig, tape(hi - 1), lam[cur], g,
-           m);
+    if (!top_lambda_done)
+        launch(ctx, "stout_lambda", lambda_kernel, grid, block, 0, level(hi - 1), level(hi), Sig, tape(hi - 1), lam[cur],
+               g, m);
     for (int k = hi; k > lo; --k) {
         const double2* U = level(k - 1);
         const double2* T = tape(k - 1);

@@ -72,12 +72,15 @@ __global__ void LT_LINK_KERNEL bk_stout_tape(const double2* __restrict__ Uin, do
     body_stout_tape(Uin + off, Uout + off, T + alg_offset(off), g, m, lb, rho);
 }
 
+template <bool LAMBDA>
 __global__ void LT_LINK_KERNEL bk_clover_deriv_all(const double2* __restrict__ U, const double2* __restrict__ C,
-                                                   double2* __restrict__ F, BatchGrid bg, const double* __restrict__ scale)
+                                                   double2* __restrict__ F, BatchGrid bg, const double* __restrict__ scale,
+                                                   SigmaNext next)
 {
     extern __shared__ double2 acc[];
     LT_BATCH_LINK_PROLOGUE
-    body_clover_deriv_all(U + off, C + clover_offset(off), F + off, g, m, lb, scale[r], acc);
+    if (LAMBDA) next = SigmaNext{next.U + off, next.T + alg_offset(off), next.Lam + alg_offset(off)};
+    body_clover_deriv_all<LAMBDA>(U + off, C + clover_offset(off), F + off, g, m, lb, scale[r], acc, next);
 }
 
 __global__ void LT_LINK_KERNEL bk_lambda(const double2* __restrict__ U, const double2* __restrict__ Uprev,
@@ -388,20 +391,26 @@ void kb_clover_charge(lt_ctx ctx, const BatchGrid& bg, int total_sb, const doubl
 }
 
 void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* C, double2* F,
-                     const double* scale)
+                     const double* scale, const SigmaNext* next)
 {
     const dim3 grid(total_lb), block(link_threads(bg, total_lb));
-    launch(ctx, "batch_clover_deriv", bk_clover_deriv_all, grid, block, 18 * block.x * sizeof(double2), U, C, F, bg, scale);
+    const size_t lds = 18 * block.x * sizeof(double2);
+    if (next)
+        launch(ctx, "batch_clover_deriv", bk_clover_deriv_all<true>, grid, block, lds, U, C, F, bg, scale, *next);
+    else
+        launch(ctx, "batch_clover_deriv", bk_clover_deriv_all<false>, grid, block, lds, U, C, F, bg, scale, SigmaNext{});
 }
 
 void kb_stout_recursion_run(lt_ctx ctx, const BatchGrid& bg, int total_lb, const LevelFn& level, const LevelFn& tape,
-                            int hi, int lo, double2* Sig, double2* const lam[2], double rho, const MomentumKick* kick)
+                            int hi, int lo, double2* Sig, double2* const lam[2], double rho, const MomentumKick* kick,
+                            bool top_lambda_done)
 {
     const dim3 grid(total_lb), block(link_threads(bg, total_lb));
     const size_t lds = 18 * block.x * sizeof(double2);
     int cur = 0;
-    launch(ctx, "batch_stout_lambda", bk_lambda, grid, block, 0, level(hi - 1), level(hi), Sig, tape(hi - 1), lam[cur],
-           bg);
+    if (!top_lambda_done)
+        launch(ctx, "batch_stout_lambda", bk_lambda, grid, block, 0, level(hi - 1), level(hi), Sig, tape(hi - 1),
+               lam[cur], bg);
     for (int k = hi; k > lo; --k) {
         const double2* U = level(k - 1);
         const double2* T = tape(k - 1);

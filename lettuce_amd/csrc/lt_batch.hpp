@@ -75,12 +75,13 @@ void kb_stout_tape(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2*
 // clover planes, charge-density partials and per-replica finish into qraw
 void kb_clover_charge(lt_ctx ctx, const BatchGrid& bg, int total_sb, const double2* U, double2* C, double* part,
                       double* qraw);
+// next: go on with the Lambda of the top stout level (see k_clover_deriv)
 void kb_clover_deriv(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2* U, const double2* C, double2* F,
-                     const double* scale);
+                     const double* scale, const SigmaNext* next = nullptr);
 // the stout-force recursion of levels hi..lo+1 (see k_stout_recursion_run)
 void kb_stout_recursion_run(lt_ctx ctx, const BatchGrid& bg, int total_lb, const LevelFn& level, const LevelFn& tape,
                             int hi, int lo, double2* Sig, double2* const lam[2], double rho,
-                            const MomentumKick* kick = nullptr);
+                            const MomentumKick* kick = nullptr, bool top_lambda_done = false);
 void kb_plaquette(lt_ctx ctx, const BatchGrid& bg, int total_sb, const double2* U, double* part, double* out);
 void kb_kinetic(lt_ctx ctx, const BatchGrid& bg, int total_kb, const double2* P, double* part, double* out);
 // scalar stages (one thread per replica)

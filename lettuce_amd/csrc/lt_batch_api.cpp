@@ -167,15 +167,19 @@ void enqueue_step(lt_batch b, const StepKey& p, int n_sub)
                 // CalculateTopologicalForce (hmc_metadynamics.hpp:374-436)
                 forward(b, bg);
                 lt::kb_dvdq(ctx, bg, st, bias, sub ? isub++ : -1);
-                lt::kb_clover_deriv(ctx, bg, b->total_lb, b->levels[b->n_levels], b->clover, b->F, st.dvdq);
+                const int n = b->n_levels;
+                // the clover derivative goes on with the Lambda of the top level
+                const lt::SigmaNext top{n > 0 ? b->levels[n - 1] : nullptr, n > 0 ? b->tapes[n - 1] : nullptr,
+                                        b->lambda[0]};
+                lt::kb_clover_deriv(ctx, bg, b->total_lb, b->levels[n], b->clover, b->F, st.dvdq, n > 0 ? &top : nullptr);
                 // KICK_BOTH: UpdateMomenta applied by the last kernel of the force (level 0 = U)
                 const lt::MomentumKick mk{b->P, e * p.beta / 6.0, e};
-                const bool fused = kick == lt::KICK_BOTH && b->n_levels > 0;
-                if (b->n_levels > 0)
+                const bool fused = kick == lt::KICK_BOTH && n > 0;
+                if (n > 0)
                     lt::kb_stout_recursion_run(
                         ctx, bg, b->total_lb, [&](int j) -> const double2* { return b->levels[j]; },
-                        [&](int j) -> const double2* { return b->tapes[j]; }, b->n_levels, 0, b->F, b->lambda, b->rho,
-                        fused ? &mk : nullptr);
+                        [&](int j) -> const double2* { return b->tapes[j]; }, n, 0, b->F, b->lambda, b->rho,
+                        fused ? &mk : nullptr, true);
                 if (fused) return;
             }
             if (kick == lt::KICK_BOTH)   // UpdateMomenta (:441-459)

@@ -333,9 +333,16 @@ __device__ __forceinline__ void lds_acc(double2* acc, int n, const M3& a, int fi
     }
     asm volatile("" ::: "memory");
 }
+__device__ __forceinline__ void lambda_link(const double2* __restrict__ U, const M3& Lprev, const M3& sig,
+                                            double2* __restrict__ Sig, const double2* __restrict__ T,
+                                            double2* __restrict__ Lam, const Geom& g, int s, int mu, int par);
+// LAMBDA: the force is the Sigma of the top stout level; instead of storing it the kernel
+// goes on with that level's Lambda (lambda_link, next: the level below, its tape and the
+// Lambda buffer), as the first launch of the stout-force recursion would.
+template <bool LAMBDA>
 __device__ __forceinline__ void body_clover_deriv_all(const double2* __restrict__ U, const double2* __restrict__ C,
                                                       double2* __restrict__ F, const Geom& g, const LinkMap& m, int b,
-                                                      double scale_factor, double2* acc)
+                                                      double scale_factor, double2* acc, const SigmaNext& next)
 {
     int mu, par, c[4], s;
     if (!map_link_b(g, m, b, mu, par, c, s)) return;
@@ -357,8 +364,12 @@ __device__ __forceinline__ void body_clover_deriv_all(const double2* __restrict_
 #pragma unroll
     for (int e = 0; e < 9; ++e) a.m[e] = cd{acc[e * m.nthreads].x, acc[e * m.nthreads].y};
     const double pref = -2.0 / (512.0 * 3.14159265358979323846 * 3.14159265358979323846);
-    a = scale(scale_factor * pref, algebra(mul(ldu(U, g, s, mu, par), a)));
-    st_link(F, g, s, mu, par, a);
+    const M3 L = ldu(U, g, s, mu, par);
+    a = scale(scale_factor * pref, algebra(mul(L, a)));
+    if (LAMBDA)
+        lambda_link(next.U, L, a, F, next.T, next.Lam, g, s, mu, par);
+    else
+        st_link(F, g, s, mu, par, a);
 }
 
 // Sigma' = Uprev^+ Sigma (stored in place) and

@@ -23,8 +23,10 @@ double k_kinetic(lt_field P);
 void k_stout_tape(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in, double2* out, double2* tape,
                   double rho);
 double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C);
+// next: the force is the Sigma of the top stout level of a recursion; go on with its
+// Lambda (the level below, its tape, the Lambda buffer) instead of only storing it
 void k_clover_deriv(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* U, const double2* C, double2* F,
-                    double scale_factor);
+                    double scale_factor, const SigmaNext* next = nullptr);
 
 }  // namespace lt
 
@@ -324,11 +326,16 @@ void cv_backward(lt_cv cv, double dvdq, double2* F, const lt::MomentumKick* kick
 {
     const int n = cv->n_levels;
     if (cv->interval == 0) {
-        lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[n], cv->clover, F, dvdq);
-        if (n > 0)
-            lt::k_stout_recursion_run(
-                cv->ctx, cv->g, cv->map, [&](int j) -> const double2* { return cv->levels[j]; },
-                [&](int j) -> const double2* { return cv->tapes[j]; }, n, 0, F, cv->lambda, cv->rho, kick);
+        if (n == 0) {
+            lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[0], cv->clover, F, dvdq);
+            return;
+        }
+        // the clover derivative goes on with the Lambda of the top level
+        const lt::SigmaNext top{cv->levels[n - 1], cv->tapes[n - 1], cv->lambda[0]};
+        lt::k_clover_deriv(cv->ctx, cv->g, cv->map, cv->levels[n], cv->clover, F, dvdq, &top);
+        lt::k_stout_recursion_run(
+            cv->ctx, cv->g, cv->map, [&](int j) -> const double2* { return cv->levels[j]; },
+            [&](int j) -> const double2* { return cv->tapes[j]; }, n, 0, F, cv->lambda, cv->rho, kick, true);
         return;
     }
     lt::k_clover_deriv(cv->ctx, cv->g, cv->map, n ? cv->top : cv->input, cv->clover, F, dvdq);

@@ -217,10 +217,11 @@ typedef std::function<const double2*(int)> LevelFn;
 // hi down to level lo: the Lambda of level hi, then for k = hi..lo+1 the sigma update of
 // level k, fused with the Lambda of level k-1 while k-1 > lo (the two Lambda buffers
 // alternate); Sig holds the force, propagated in place.  kick: applied by the update of
-// level 1 instead of storing the level-0 force (lo == 0 only).
+// level 1 instead of storing the level-0 force (lo == 0 only).  top_lambda_done: the
+// Lambda of level hi is already in lam[0] (the clover derivative made it).
 void k_stout_recursion_run(lt_ctx ctx, const Geom& g, const LinkMap& m, const LevelFn& level, const LevelFn& tape,
                            int hi, int lo, double2* Sig, double2* const lam[2], double rho,
-                           const MomentumKick* kick = nullptr);
+                           const MomentumKick* kick = nullptr, bool top_lambda_done = false);
 // The level below in a stout-force recursion step fused with its Lambda: its input
 // field, its exponent tape and the Lambda buffer to write.
 struct SigmaNext {
