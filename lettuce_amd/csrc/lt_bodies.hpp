@@ -240,26 +240,11 @@ __device__ __forceinline__ int dn_rt(const Geom& g, int s, int cm, int d)
     return (cm > 0) ? s - st : s + (L - 1) * st;
 }
 
-// Parking slot: the first of the two products of a half waits in LDS (9 entries,
-// stride n) while the second is formed, so it holds no registers meanwhile; the empty
-// asm statements with memory clobbers keep the compiler from moving it back.  park is
-// never null in the kernels: testing it anyway puts the parking in a branch of its own,
-// and the compiler does not hoist the second product's link loads above a branch (19
-// spilled VGPRs in clover_deriv_all_kernel without the tests, 2 with them).
-__device__ __forceinline__ void park_store(double2* park, int n, const M3& a)
-{
-#pragma unroll
-    for (int e = 0; e < 9; ++e) park[e * n] = make_double2(a.m[e].re, a.m[e].im);
-    asm volatile("" ::: "memory");
-}
-__device__ __forceinline__ M3 park_load(const double2* park, int n)
-{
-    asm volatile("" ::: "memory");
-    M3 a;
-#pragma unroll
-    for (int e = 0; e < 9; ++e) a.m[e] = cd{park[e * n].x, park[e * n].y};
-    return a;
-}
+// Parking slot (park_store / park_load): the first of the two products of a half waits
+// in LDS while the second is formed, so it holds no registers meanwhile.  park is never
+// null in the kernels: testing it anyway puts the parking in a branch of its own, and the
+// compiler does not hoist the second product's link loads above a branch (19 spilled
+// VGPRs in clover_deriv_all_kernel without the tests, 2 with them).
 
 __device__ __forceinline__ M3 cderiv_upper(const double2* __restrict__ U, const double2* __restrict__ C, const Geom& g,
                                            int s, const int c[4], int par, int MU, int NU, int RHO, int SIGMA,

@@ -1231,6 +1231,24 @@ __device__ __forceinline__ int shift2(const Geom& g, int s, const int c[4], int 
            (wrap_add(cn, b, pick4(g.L, nu)) - cn) * pick4(g.st, nu);
 }
 
+// Parking: a matrix waits in LDS (9 entries at park[e * n], one slot per lane) while
+// other work runs, so it holds no registers meanwhile; the empty asm statements with
+// memory clobbers keep the compiler from carrying it in registers anyway.
+__device__ __forceinline__ void park_store(double2* park, int n, const M3& a)
+{
+#pragma unroll
+    for (int e = 0; e < 9; ++e) park[e * n] = make_double2(a.m[e].re, a.m[e].im);
+    asm volatile("" ::: "memory");
+}
+__device__ __forceinline__ M3 park_load(const double2* park, int n)
+{
+    asm volatile("" ::: "memory");
+    M3 a;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a.m[e] = cd{park[e * n].x, park[e * n].y};
+    return a;
+}
+
 // Rectangular<2>::StapleRect (rectangular_action.hpp:186-225): for nu = mu+1, mu+2,
 // mu+3 (mod 4) the six 1x2 staples of link (x, mu) -- orthogonal to it (two), extending
 // it forward (two), backward (two) -- in the reference's grouping and order.  Every
@@ -1239,10 +1257,18 @@ __device__ __forceinline__ int shift2(const Geom& g, int s, const int c[4], int 
 // forward pair is taken as a U^+ + b U^+ and the backward pair as U^+ a + U^+ b (the
 // reference's (a + b) U^+ and U^+ (a + b)), so every accumulated term is a unitary chain.
 __device__ __forceinline__ M3 staple_rect(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
-                                          int par)
+                                          int par, double2* park = nullptr, int n = 0)
 {
     const int q = par ^ 1;
     M3 st = m3_zero();
+    // park (optional, see park_store): the sum waits in LDS between the terms, so that
+    // one term at a time holds registers; the same additions in the same order
+    if (park) park_store(park, n, st);
+    auto add_term = [&](const M3& r) {
+        if (park) st = park_load(park, n);
+        acc_su3_r01(st, r);
+        if (park) park_store(park, n, st);
+    };
 #pragma unroll 1
     for (int off = 1; off < 4; ++off) {
         const int nu = (mu + off) & 3;
@@ -1255,12 +1281,12 @@ __device__ __forceinline__ M3 staple_rect(const double2* __restrict__ U, const G
             r = mul_r01(r, ldu(U, g, nupp, mu, par));
             r = mul_nd_r01(r, ldu(U, g, mup_nup, nu, par));
             r = mul_nd_r01(r, ldu(U, g, mup, nu, q));
-            acc_su3_r01(st, r);
+            add_term(r);
             r = mul_dd_r01(ldu(U, g, nud, nu, q), ldu(U, g, nudd, nu, par));
             r = mul_r01(r, ldu(U, g, nudd, mu, par));
             r = mul_r01(r, ldu(U, g, mup_nudd, nu, q));
             r = mul_r01(r, ldu(U, g, mup_nud, nu, par));
-            acc_su3_r01(st, r);
+            add_term(r);
         }
         // forward: [U_nu(x) U_mu(x+nu) U_mu(x+mu+nu) U_nu(x+2mu)^+ + mirror] U_mu(x+mu)^+
         {
@@ -1270,12 +1296,12 @@ __device__ __forceinline__ M3 staple_rect(const double2* __restrict__ U, const G
             r = mul_r01(r, ldu(U, g, mup_nup, mu, par));
             r = mul_nd_r01(r, ldu(U, g, mupp, nu, par));
             r = mul_nd_r01(r, ump);
-            acc_su3_r01(st, r);
+            add_term(r);
             r = mul_dn_r01(ldu(U, g, nud, nu, q), ldu(U, g, nud, mu, q));
             r = mul_r01(r, ldu(U, g, mup_nud, mu, par));
             r = mul_r01(r, ldu(U, g, mupp_nud, nu, q));
             r = mul_nd_r01(r, ump);
-            acc_su3_r01(st, r);
+            add_term(r);
         }
         // backward: U_mu(x-mu)^+ [U_nu(x-mu) U_mu(x-mu+nu) U_mu(x+nu) U_nu(x+mu)^+ + mirror]
         {
@@ -1285,30 +1311,36 @@ __device__ __forceinline__ M3 staple_rect(const double2* __restrict__ U, const G
             r = mul_r01(r, ldu(U, g, mud_nup, mu, par));
             r = mul_r01(r, ldu(U, g, nup, mu, q));
             r = mul_nd_r01(r, ldu(U, g, mup, nu, q));
-            acc_su3_r01(st, r);
+            add_term(r);
             r = mul_dd_r01(umd, ldu(U, g, mud_nud, nu, par));
             r = mul_r01(r, ldu(U, g, mud_nud, mu, par));
             r = mul_r01(r, ldu(U, g, nud, mu, q));
             r = mul_r01(r, ldu(U, g, mup_nud, nu, par));
-            acc_su3_r01(st, r);
+            add_term(r);
         }
     }
+    if (park) st = park_load(park, n);
     return st;
 }
 
 // Rectangular<R>::Staple (rectangular_action.hpp:232-242)
 __device__ __forceinline__ M3 staple_action(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int mu,
-                                            int par, double c_plaq, double c_rect)
+                                            int par, double c_plaq, double c_rect, double2* park = nullptr, int n = 0)
 {
     const M3 p = staple(U, g, s, c, mu);
     if (c_rect == 0.0) return c_plaq == 1.0 ? p : scale(c_plaq, p);
-    return add(scale(c_plaq, p), scale(c_rect, staple_rect(U, g, s, c, mu, par)));
+    return add(scale(c_plaq, p), scale(c_rect, staple_rect(U, g, s, c, mu, par, park, n)));
 }
 
 // Sum of the four clover leaves of plane (mu, nu) at site s (leaf orientation of
 // the reference's CalculateClover<1,1>); leaf I (the plaquette) is returned too.
+
+// park (optional, see park_store): the running sum of the leaves waits in LDS while the
+// next leaf is formed, and the leaves read their links afresh, so that one leaf at a
+// time holds registers (the tests of park are branches the compiler does not hoist the
+// next leaf's loads above); the same operations in the same order either way.
 __device__ __forceinline__ M3 clover_plane(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int par,
-                                           int mu, int nu, double& retr_plaq)
+                                           int mu, int nu, double& retr_plaq, double2* park = nullptr, int n = 0)
 {
     // every leaf is a product of four links, an SU(3) matrix: rows 0 and 1 are
     // multiplied out along the chain, row 2 is conj(row0 x row1)
@@ -1316,21 +1348,29 @@ __device__ __forceinline__ M3 clover_plane(const double2* __restrict__ U, const 
     const int smu = up(g, s, c[mu], mu), snu = up(g, s, c[nu], nu);
     const int smd = dn(g, s, c[mu], mu), snd = dn(g, s, c[nu], nu);
     const int smd_nu = up(g, smd, c[nu], nu), smd_nd = dn(g, smd, c[nu], nu), smu_nd = up(g, snd, c[mu], mu);
-    const M3 Umu = ldu(U, g, s, mu, par), Unu = ldu(U, g, s, nu, par);
     // leaf I: U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-    M3 C = mul_nd_r01(mul_nd_r01(mul_r01(Umu, ldu(U, g, smu, nu, q)), ldu(U, g, snu, mu, q)), Unu);
+    M3 C = mul_nd_r01(mul_nd_r01(mul_r01(ldu(U, g, s, mu, par), ldu(U, g, smu, nu, q)), ldu(U, g, snu, mu, q)),
+                      ldu(U, g, s, nu, par));
     third_row(C);
     retr_plaq = C.m[0].re + C.m[4].re + C.m[8].re;
+    if (park) park_store(park, n, C);
     // leaf II: U_nu(x) U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
-    const M3 Umu_md = ldu(U, g, smd, mu, q);
-    acc_su3_r01(C, mul_r01(mul_nd_r01(mul_nd_r01(Unu, ldu(U, g, smd_nu, mu, par)), ldu(U, g, smd, nu, q)), Umu_md));
+    M3 t = mul_r01(mul_nd_r01(mul_nd_r01(ldu(U, g, s, nu, par), ldu(U, g, smd_nu, mu, par)), ldu(U, g, smd, nu, q)),
+                   ldu(U, g, smd, mu, q));
+    if (park) C = park_load(park, n);
+    acc_su3_r01(C, t);
+    if (park) park_store(park, n, C);
     // leaf III: U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu)
-    acc_su3_r01(C, mul_r01(mul_r01(mul_dd_r01(Umu_md, ldu(U, g, smd_nd, nu, par)), ldu(U, g, smd_nd, mu, par)),
-                           ldu(U, g, snd, nu, q)));
+    t = mul_r01(mul_r01(mul_dd_r01(ldu(U, g, smd, mu, q), ldu(U, g, smd_nd, nu, par)), ldu(U, g, smd_nd, mu, par)),
+                ldu(U, g, snd, nu, q));
+    if (park) C = park_load(park, n);
+    acc_su3_r01(C, t);
+    if (park) park_store(park, n, C);
     // leaf IV: U_nu(x-nu)^+ U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^+
-    acc_su3_r01(C, mul_nd_r01(mul_r01(mul_dn_r01(ldu(U, g, snd, nu, q), ldu(U, g, snd, mu, q)),
-                                      ldu(U, g, smu_nd, nu, par)),
-                              Umu));
+    t = mul_nd_r01(mul_r01(mul_dn_r01(ldu(U, g, snd, nu, q), ldu(U, g, snd, mu, q)), ldu(U, g, smu_nd, nu, par)),
+                   ldu(U, g, s, mu, par));
+    if (park) C = park_load(park, n);
+    acc_su3_r01(C, t);
     return C;
 }
 

@@ -49,17 +49,23 @@ __device__ __forceinline__ void block_sum3(double v0, double v1, double v2, doub
     }
 }
 
+// lds: two parking slots per lane (see park_store; entries e * BS + lane and, for F_a
+// while F_b is formed, (9 + e) * BS + lane); LDS_BYTES per block
+constexpr size_t LDS_BYTES = 18 * BS * sizeof(double2);
 template <int A0, int A1, int B0, int B1>
 __device__ __forceinline__ void pair_terms(const double2* __restrict__ U, const Geom& g, int s, const int c[4], int par,
-                                           double& p, double& q, double& e)
+                                           double& p, double& q, double& e, double2* lds)
 {
     double pl;
-    M3 Fa = ah2(clover_plane(U, g, s, c, par, A0, A1, pl));
+    double2* slot = lds + threadIdx.x;
+    M3 Fa = ah2(clover_plane(U, g, s, c, par, A0, A1, pl, slot, BS));
     p += pl;
     e += energy_term(Fa);
-    M3 Fb = ah2(clover_plane(U, g, s, c, par, B0, B1, pl));
+    if (lds) park_store(slot + 9 * BS, BS, Fa);
+    M3 Fb = ah2(clover_plane(U, g, s, c, par, B0, B1, pl, slot, BS));
     p += pl;
     e += energy_term(Fb);
+    if (lds) Fa = park_load(slot + 9 * BS, BS);
     q += retr_mul(Fa, Fb);
 }
 
@@ -69,9 +75,12 @@ __device__ __forceinline__ void pair_terms(const double2* __restrict__ U, const 
 // sums of the plaquette, raw q and energy go to part[(pair, t, chunk)].  The pairs
 // (01,23), (02,13), (03,12) enter q = tr F01F23 - tr F02F13 + tr F03F12; one pair
 // per thread keeps registers low.
-__global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, Geom g, int Vs, int nchunk,
-                                                 double* part)
+// two waves per SIMD: the parked clover planes fit 256 VGPRs (one wave and 330
+// registers without parking)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
+obs_kernel(const double2* __restrict__ U, Geom g, int Vs, int nchunk, double* part)
 {
+    extern __shared__ double2 lds[];
     int b = xcd_block(blockIdx.x, gridDim.x);
     const int pair = b % 3;
     b /= 3;
@@ -85,12 +94,12 @@ __global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, 
         coords_of(g, s, c);
         const int par = parity_of(c);
         if (pair == 0) {
-            pair_terms<0, 1, 2, 3>(U, g, s, c, par, p, q, e);
+            pair_terms<0, 1, 2, 3>(U, g, s, c, par, p, q, e, lds);
         } else if (pair == 1) {
-            pair_terms<0, 2, 1, 3>(U, g, s, c, par, p, q, e);
+            pair_terms<0, 2, 1, 3>(U, g, s, c, par, p, q, e, lds);
             q = -q;
         } else {
-            pair_terms<0, 3, 1, 2>(U, g, s, c, par, p, q, e);
+            pair_terms<0, 3, 1, 2>(U, g, s, c, par, p, q, e, lds);
         }
     }
     block_sum3(p, q, e, part + 3 * (((size_t)pair * g.L[0] + t) * nchunk + chunk));
@@ -99,17 +108,19 @@ __global__ void __launch_bounds__(BS) obs_kernel(const double2* __restrict__ U, 
 // TopChargeClover's summand per site (topological_charge.hpp:184-199): one thread
 // per site (lexicographic order) evaluates all three plane pairs and writes
 // -1/(256 pi^2) (tr F01F23 - tr F02F13 + tr F03F12), F = C - C^+.
-__global__ void __launch_bounds__(BS) qdensity_kernel(const double2* __restrict__ U, Geom g, double* __restrict__ q_x)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
+qdensity_kernel(const double2* __restrict__ U, Geom g, double* __restrict__ q_x)
 {
+    extern __shared__ double2 lds[];
     const int s = blockIdx.x * BS + threadIdx.x;
     if (s >= g.V) return;
     int c[4];
     coords_of(g, s, c);
     const int par = parity_of(c);
     double p = 0.0, e = 0.0, q01 = 0.0, q02 = 0.0, q03 = 0.0;
-    pair_terms<0, 1, 2, 3>(U, g, s, c, par, p, q01, e);
-    pair_terms<0, 2, 1, 3>(U, g, s, c, par, p, q02, e);
-    pair_terms<0, 3, 1, 2>(U, g, s, c, par, p, q03, e);
+    pair_terms<0, 1, 2, 3>(U, g, s, c, par, p, q01, e, lds);
+    pair_terms<0, 2, 1, 3>(U, g, s, c, par, p, q02, e, lds);
+    pair_terms<0, 3, 1, 2>(U, g, s, c, par, p, q03, e, lds);
     q_x[s] = (-1.0 / (256.0 * M_PI * M_PI)) * (q01 - q02 + q03);
 }
 
@@ -347,7 +358,7 @@ void k_observables(lt_field u, double* host_out)
     const unsigned nchunk = nblocks(Vs, BS);
     double* part = ctx->scratch_for(9 * (size_t)nchunk * nt + 3 * (size_t)nt);
     double* fin = part + 9 * (size_t)nchunk * nt;
-    launch(ctx, "observables", obs_kernel, dim3(3 * nchunk * nt), dim3(BS), 0, (const double2*)u->d, g, Vs,
+    launch(ctx, "observables", obs_kernel, dim3(3 * nchunk * nt), dim3(BS), LDS_BYTES, (const double2*)u->d, g, Vs,
            (int)nchunk, part);
     launch(ctx, "observables_finish", obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, (const double*)part,
            (int)nchunk, nt, fin);
@@ -436,7 +447,7 @@ void k_observables_noalloc(lt_field u, double* host_out)
     double* out = fin + 3 * (size_t)nt;
     // every partial and every per-timeslice sum is written before it is read: nothing
     // has to start at zero
-    hipLaunchKernelGGL(obs_kernel, dim3(3 * nchunk * nt), dim3(BS), 0, ctx->stream, (const double2*)u->d, g, Vs,
+    hipLaunchKernelGGL(obs_kernel, dim3(3 * nchunk * nt), dim3(BS), LDS_BYTES, ctx->stream, (const double2*)u->d, g, Vs,
                        (int)nchunk, part);
     LT_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, ctx->stream, (const double*)part,
@@ -454,7 +465,7 @@ void k_topcharge_density(lt_field u, double* host_q)
     lt_ctx ctx = u->ctx;
     const Geom& g = u->g;
     double* d = ctx->scratch_for((size_t)g.V);
-    launch(ctx, "topcharge_density", qdensity_kernel, dim3(nblocks(g.V, BS)), dim3(BS), 0, (const double2*)u->d, g, d);
+    launch(ctx, "topcharge_density", qdensity_kernel, dim3(nblocks(g.V, BS)), dim3(BS), LDS_BYTES, (const double2*)u->d, g, d);
     LT_HIP_CHECK(hipMemcpyAsync(host_q, d, (size_t)g.V * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }

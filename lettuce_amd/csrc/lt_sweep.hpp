@@ -179,9 +179,9 @@ __device__ __forceinline__ void update_link(double2* __restrict__ U, const Geom&
 template <bool HEATBATH>
 __device__ __forceinline__ void update_link_action(double2* __restrict__ U, const Geom& g, int s, const int c[4],
                                                    int mu, int par, double prefactor, uint64_t seed, uint64_t counter,
-                                                   double c_plaq, double c_rect)
+                                                   double c_plaq, double c_rect, double2* park = nullptr, int n = 0)
 {
-    const M3 st = staple_action(U, g, s, c, mu, par, c_plaq, c_rect);
+    const M3 st = staple_action(U, g, s, c, mu, par, c_plaq, c_rect, park, n);
     M3 L = ldu(U, g, s, mu, par);
 #pragma unroll
     for (int sub = 0; sub < 3; ++sub) {

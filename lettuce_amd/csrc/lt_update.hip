@@ -183,16 +183,18 @@ __global__ void upd_advance_kernel(uint64_t* state, int n_hb)
     }
 }
 
-__global__ void __launch_bounds__(BS) staple_action_kernel(const double2* __restrict__ U, double2* __restrict__ out,
-                                                           Geom g, double c_plaq, double c_rect)
+// the rectangle staple's sum waits in LDS between its terms (9 x 16 bytes per lane)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
+staple_action_kernel(const double2* __restrict__ U, double2* __restrict__ out, Geom g, double c_plaq, double c_rect)
 {
+    extern __shared__ double2 lds[];
     const long long id = (long long)blockIdx.x * BS + threadIdx.x;
     if (id >= 4LL * g.V) return;
     const int s = (int)(id >> 2), mu = (int)(id & 3);
     int c[4];
     coords_of(g, s, c);
     const int par = parity_of(c);
-    st_link(out, g, s, mu, par, staple_action(U, g, s, c, mu, par, c_plaq, c_rect));
+    st_link(out, g, s, mu, par, staple_action(U, g, s, c, mu, par, c_plaq, c_rect, lds + threadIdx.x, BS));
 }
 
 __global__ void LT_LINK_KERNEL staple_field_kernel(const double2* __restrict__ U, double2* __restrict__ out, Geom g,
@@ -269,8 +271,8 @@ void sweeps_action(lt_field u, int n, double prefactor, double c_plaq, double c_
 
 void k_staple_field_action(lt_field u, lt_field out, double c_plaq, double c_rect)
 {
-    launch(u->ctx, "staple_action", staple_action_kernel, dim3(nblocks(4LL * u->g.V, BS)), dim3(BS), 0,
-           (const double2*)u->d, out->d, u->g, c_plaq, c_rect);
+    launch(u->ctx, "staple_action", staple_action_kernel, dim3(nblocks(4LL * u->g.V, BS)), dim3(BS),
+           9 * BS * sizeof(double2), (const double2*)u->d, out->d, u->g, c_plaq, c_rect);
 }
 
 void k_or_sweep_action(lt_field u, int n, double c_plaq, double c_rect)
