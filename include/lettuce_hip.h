@@ -151,12 +151,15 @@ int lt_heatbath_sweep(lt_field u, double beta, int n_sweep, int schedule);
  * once as a hipGraph (the Philox counter and the output slot live in device memory)
  * and replayed per update, for small lattices whose sweeps are launch-bound; the field
  * equals that of the same lt_heatbath_sweep / lt_or_sweep calls bit for bit, and the
- * field's counter advances by n_hb per update.  The plaquette sums come from a
- * per-site reduction (they agree with lt_plaquette_sum to rounding). */
+ * field's counter advances by n_hb per update.  The plaquette sums are lt_plaquette_sum's
+ * (the same terms and reduction tree, bit for bit). */
 int lt_update_sweeps(lt_field u, double beta, int n_hb, int n_or, int schedule, int n_update, double *plaq_sum);
 
 /* ---- observables ---------------------------------------------------------- */
-/* PlaquetteSum (plaquette.hpp:26-43) */
+/* PlaquetteSum (plaquette.hpp:26-43): sum over sites and the six planes of Re tr P, one
+ * link (x, mu) per GPU lane with the planes (mu, mu+1 mod 4) and, for mu < 2, (mu, mu+2),
+ * in the tile order of the field's link kernels; agrees with the plaquette of
+ * lt_observables to rounding */
 int lt_plaquette_sum(lt_field u, double *out);
 /* WilsonAction::Action / Rectangular<1>::Action (wilson_action.hpp:23-41) */
 int lt_wilson_action(lt_field u, double beta, double *out);

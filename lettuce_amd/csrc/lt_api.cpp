@@ -796,10 +796,11 @@ int lt_polyakov_loop(lt_field u, double* re, double* im)
 int lt_plaquette_sum(lt_field u, double* out)
 {
     if (!u || !out) return fail(LT_EINVAL, "null argument");
-    std::vector<double> o(4 + 2 * (size_t)u->g.L[0]);
-    int rc = lt_observables(u, o.data());
-    if (rc == LT_OK) *out = o[0];
-    return rc;
+    return guarded([&] {
+        use_device(u->ctx);
+        *out = lt::k_plaquette_planes(u);
+        return LT_OK;
+    });
 }
 
 int lt_wilson_action(lt_field u, double beta, double* out)

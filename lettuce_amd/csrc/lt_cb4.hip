@@ -60,11 +60,11 @@ sweep_action_kernel(double2* __restrict__ U, Geom g, int mu, int colour, double 
 }
 
 // PlaquetteSum block partials of a captured four-colour update (the field is in the
-// split order inside the graph); same per-site terms and tree as the plain kernel
-__global__ void __launch_bounds__(RBS) cb4_plaq_kernel(const double2* __restrict__ U, Geom g, double* part)
+// split order inside the graph); the same terms, launch and partials as the
+// plain-order kernel
+__global__ void LT_LINK_KERNEL cb4_plaq_kernel(const double2* __restrict__ U, Geom g, LinkMap m, double* part)
 {
-    const int s = blockIdx.x * RBS + threadIdx.x;
-    block_sum_to<RBS>(s < g.V ? body_plaq_site(U, g, s) : 0.0, part + blockIdx.x);
+    plaq_link_partials(U, g, m, part);
 }
 
 // In-place reorder of rows 0 and 1 of every link between the plain cb order and the
@@ -124,9 +124,10 @@ void k_cb4_colour_action(lt_field u, bool heatbath, int mu, int colour, double p
                prefactor, u->seed, counter, c_plaq, c_rect);
 }
 
-void k_cb4_plaq_partials(lt_field u, double* part, unsigned nb)
+void k_cb4_plaq_partials(lt_field u, double* part)
 {
-    launch(u->ctx, "plaquette", cb4_plaq_kernel, dim3(nb), dim3(RBS), 0, (const double2*)u->d, u->g, part);
+    launch(u->ctx, "plaquette", cb4_plaq_kernel, dim3(8 * map_blocks(u->map)), dim3(u->map.nthreads), 0,
+           (const double2*)u->d, u->g, u->map, part);
 }
 
 }  // namespace lt

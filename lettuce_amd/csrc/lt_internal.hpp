@@ -279,7 +279,7 @@ void k_cb4_colour(lt_field u, bool heatbath, int mu, int colour, double prefacto
 void k_cb4_colour_graph_hb(lt_field u, int mu, int colour, double prefactor, const uint64_t* state, int k);
 void k_cb4_colour_action(lt_field u, bool heatbath, int mu, int colour, double prefactor, uint64_t counter,
                          double c_plaq, double c_rect, const char* name);
-void k_cb4_plaq_partials(lt_field u, double* part, unsigned nb);
+void k_cb4_plaq_partials(lt_field u, double* part);   // one partial per block of u->map's launch
 // n_update x (n_hb heatbath + n_or overrelaxation sweeps) as one graph replayed per
 // update, PlaquetteSum after each into plaq_host (may be null); and its teardown
 void k_update_graph(lt_field u, double beta, int n_hb, int n_or, int schedule, int n_update, double* plaq_host);
@@ -295,6 +295,9 @@ void k_staple_field_action(lt_field u, lt_field out, double c_plaq, double c_rec
 void k_or_sweep_action(lt_field u, int n, double c_plaq, double c_rect);
 void k_heatbath_sweep_action(lt_field u, double beta, int n, double c_plaq, double c_rect);
 void k_observables(lt_field u, double* host_out);
+// sum over sites and planes of Re tr P, the planes of a site spread over its links (the HMC
+// Hamiltonian keeps k_plaquette_sum, one site per lane, which the replica batch repeats)
+double k_plaquette_planes(lt_field u);
 // per-site clover charge density (V doubles, lexicographic order) into host memory
 void k_topcharge_density(lt_field u, double* host_q);
 // the fixed observables scratch of lt_observables_noalloc: its address on `device`

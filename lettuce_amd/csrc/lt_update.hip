@@ -162,10 +162,17 @@ __device__ __forceinline__ void block_sum1(double v, double* out)
 {
     block_sum_to<RBS>(v, out);
 }
-__global__ void __launch_bounds__(RBS) upd_plaq_kernel(const double2* __restrict__ U, Geom g, double* part)
+// plaquette sum over the field's link map, one partial per block
+__global__ void LT_LINK_KERNEL upd_plaq_kernel(const double2* __restrict__ U, Geom g, LinkMap m, double* part)
 {
-    const int s = blockIdx.x * RBS + threadIdx.x;
-    block_sum1(s < g.V ? body_plaq_site(U, g, s) : 0.0, part + blockIdx.x);
+    plaq_link_partials(U, g, m, part);
+}
+unsigned plaq_parts(const LinkMap& m) { return 8u * map_blocks(m); }
+__global__ void __launch_bounds__(RBS) plaq_sum_kernel(const double* __restrict__ part, int n, double* out)
+{
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += RBS) acc += part[i];
+    block_sum1(acc, out);
 }
 __global__ void __launch_bounds__(RBS) upd_plaq_finish_kernel(const double* __restrict__ part, int n,
                                                               const uint64_t* __restrict__ state, double* plaq)
@@ -306,8 +313,9 @@ struct UpdateGraph {
     uint64_t* state = nullptr;
     double* part = nullptr;
     double* plaq = nullptr;
-    size_t cap = 0;
-    unsigned nb = 0;
+    size_t cap = 0;        // plaquette sums held by plaq
+    unsigned part_cap = 0;  // partials held by part
+    unsigned nb = 0;        // partials of the present launch map
 
     void drop_exec()
     {
@@ -356,9 +364,10 @@ void enqueue_update(lt_field u, UpdateGraph& G, double prefactor)
     sweep_launches<false>(u, G.n_or, G.schedule, 0.0, G.schedule == LT_CB4 ? "or_cb4" : "or_cb2");
     if (G.with_plaq) {
         if (G.schedule == LT_CB4)
-            k_cb4_plaq_partials(u, G.part, G.nb);
+            k_cb4_plaq_partials(u, G.part);
         else
-            launch(ctx, "plaquette", upd_plaq_kernel, dim3(G.nb), dim3(RBS), 0, (const double2*)u->d, g, G.part);
+            launch(ctx, "plaquette", upd_plaq_kernel, dim3(8 * map_blocks(u->map)), dim3(u->map.nthreads), 0,
+                   (const double2*)u->d, g, u->map, G.part);
         launch(ctx, "reduce_finish", upd_plaq_finish_kernel, dim3(1), dim3(RBS), 0, (const double*)G.part, (int)G.nb,
                (const uint64_t*)G.state, G.plaq);
     }
@@ -373,11 +382,20 @@ void k_update_graph(lt_field u, double beta, int n_hb, int n_or, int schedule, i
     if (!u->ug) u->ug = new UpdateGraph();
     UpdateGraph& G = *u->ug;
     const int with_plaq = plaq_host ? 1 : 0;
-    if (!G.state) {
-        G.nb = nblocks(u->g.V, RBS);
-        if (hipMalloc(&G.state, 2 * sizeof(uint64_t)) != hipSuccess || hipMalloc(&G.part, G.nb * sizeof(double)) != hipSuccess)
+    if (!G.state && hipMalloc(&G.state, 2 * sizeof(uint64_t)) != hipSuccess)
+        throw LtError{LT_ENOMEM, "update graph allocation failed"};
+    // plaquette partials: one per block of the field's present link map
+    const unsigned np = plaq_parts(u->map);
+    if (np > G.part_cap) {
+        G.drop_exec();
+        if (G.part) LT_HIP_CHECK(hipFree(G.part));
+        G.part = nullptr;
+        G.part_cap = 0;
+        if (hipMalloc(&G.part, (size_t)np * sizeof(double)) != hipSuccess)
             throw LtError{LT_ENOMEM, "update graph allocation failed"};
+        G.part_cap = np;
     }
+    G.nb = np;
     if (with_plaq && (size_t)n_update > G.cap) {
         G.drop_exec();
         if (G.plaq) LT_HIP_CHECK(hipFree(G.plaq));
@@ -426,6 +444,20 @@ void k_update_graph(lt_field u, double beta, int n_hb, int n_or, int schedule, i
         LT_HIP_CHECK(hipMemcpyAsync(plaq_host, G.plaq, (size_t)n_update * sizeof(double), hipMemcpyDeviceToHost,
                                     ctx->stream));
     LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+double k_plaquette_planes(lt_field u)
+{
+    lt_ctx ctx = u->ctx;
+    const unsigned nb = plaq_parts(u->map);
+    double* part = ctx->scratch_for((size_t)nb + 1);
+    launch(ctx, "plaquette", upd_plaq_kernel, dim3(8 * map_blocks(u->map)), dim3(u->map.nthreads), 0,
+           (const double2*)u->d, u->g, u->map, part);
+    launch(ctx, "plaquette_sum", plaq_sum_kernel, dim3(1), dim3(RBS), 0, (const double*)part, (int)nb, part + nb);
+    double* h = ctx->host_for(1);
+    LT_HIP_CHECK(hipMemcpyAsync(h, part + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return h[0];
 }
 
 void k_or_sweep(lt_field u, int n, int schedule)
