@@ -466,6 +466,10 @@ def main():
     for _ in range(10):
         la.Observables(W)
     obs_ms = 100.0 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        la.PlaquetteSum(W)
+    plaq_sum_ms = 100.0 * (time.perf_counter() - t0)
     # the rest of the reference's Observables(): Symanzik action, plaquette charge,
     # Wilson loops L = 2, 4, 8 on one chain workspace, Polyakov loop
     chain = la.MatrixField(ctx, shape)
@@ -564,6 +568,7 @@ def main():
                      "link_stage_updates_per_s": 12 * V / (flow_ms * 1e-3)},
         "read_ceiling": read_ceiling,
         "observables_pass_ms": obs_ms,
+        "plaquette_sum_ms": plaq_sum_ms,
         "observables_rest_ms": obs_rest_ms,
         "lw_or_sweep_ms": lw_or_ms,
         "cb4_update_ms": cb4_update_ms,
