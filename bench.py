@@ -466,6 +466,7 @@ def main():
     for _ in range(10):
         la.Observables(W)
     obs_ms = 100.0 * (time.perf_counter() - t0)
+    la.PlaquetteSum(W)
     t0 = time.perf_counter()
     for _ in range(10):
         la.PlaquetteSum(W)
@@ -474,14 +475,18 @@ def main():
     # Wilson loops L = 2, 4, 8 on one chain workspace, Polyakov loop
     chain = la.MatrixField(ctx, shape)
     sym = la.RectangularAction(args.beta, 1.0 + 8.0 / 12.0, -1.0 / 12.0)
-    t0 = time.perf_counter()
-    for _ in range(3):
+
+    def obs_rest():
         sym.ActionNormalized(W)
         la.TopChargePlaquette(W)
         la.WilsonLoop(0, 2, True, W, chain)
         la.WilsonLoop(2, 4, False, W, chain)
         la.WilsonLoop(4, 8, False, W, chain)
         la.PolyakovLoop(W)
+    obs_rest()     # first launches of these kernels and first touch of the chain workspace
+    t0 = time.perf_counter()
+    for _ in range(3):
+        obs_rest()
     obs_rest_ms = (time.perf_counter() - t0) / 3 * 1e3
     chain.close()
     fl.close()
