@@ -242,24 +242,14 @@ __device__ __forceinline__ int dn_rt(const Geom& g, int s, int cm, int d)
 
 // Re tr of the plaquette U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ (plaquette.hpp:20-24)
 // as Re tr(A B^+) with A = U_mu(x) U_nu(x+mu), B = U_nu(x) U_mu(x+nu), both in SU(3), of
-// which only rows 0 and 1 are multiplied out.  With G_ij = row_i(A) . conj(row_j(B))
-// (i, j < 2), the row-2 term conj(A_0 x A_1) . (B_0 x B_1) equals conj(G_00 G_11 - G_01 G_10)
-// (Binet-Cauchy), so Re tr(A B^+) = Re(G_00 + G_11 + G_00 G_11 - G_01 G_10).
+// which only rows 0 and 1 are multiplied out (retr_su3_nd_r01).
 __device__ __forceinline__ double plaq_plane(const double2* __restrict__ U, const Geom& g, int s, const int c[4],
                                             int par, int mu, int nu)
 {
     const int smu = up_rt(g, s, pick4(c, mu), mu), snu = up_rt(g, s, pick4(c, nu), nu);
     const M3 A = mul_r01(ldu2(U, g, s, mu, par), ldu(U, g, smu, nu, par ^ 1));
     const M3 B = mul_r01(ldu2(U, g, s, nu, par), ldu(U, g, snu, mu, par ^ 1));
-    cd G[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            G[i][j] = dot3_acc<false, true>(cd{0.0, 0.0}, A.m[3 * i], B.m[3 * j], A.m[3 * i + 1], B.m[3 * j + 1],
-                                            A.m[3 * i + 2], B.m[3 * j + 2]);
-    const cd d = G[0][0] * G[1][1] - G[0][1] * G[1][0];
-    return G[0][0].re + G[1][1].re + d.re;
+    return retr_su3_nd_r01(A, B);
 }
 // The plaquettes of link (x, mu) in a launch over the field's link map: the planes
 // (mu, mu+1 mod 4) and, for mu < 2, (mu, mu+2) -- every plane of every site once -- in the

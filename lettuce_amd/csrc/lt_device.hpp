@@ -329,6 +329,21 @@ __device__ __forceinline__ double retr_mul(const M3& a, const M3& b)
         for (int k = 0; k < 3; ++k) s += a.m[3 * i + k].re * b.m[3 * k + i].re - a.m[3 * i + k].im * b.m[3 * k + i].im;
     return s;
 }
+// Re tr(a b^+) for a, b in SU(3) given by rows 0 and 1: with G_ij = row_i(a) . conj(row_j(b))
+// (i, j < 2) the row-2 term conj(a_0 x a_1) . (b_0 x b_1) equals conj(G_00 G_11 - G_01 G_10)
+// (Binet-Cauchy), so Re tr(a b^+) = Re(G_00 + G_11 + G_00 G_11 - G_01 G_10).
+__device__ __forceinline__ double retr_su3_nd_r01(const M3& a, const M3& b)
+{
+    cd G[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            G[i][j] = dot3_acc<false, true>(cd{0.0, 0.0}, a.m[3 * i], b.m[3 * j], a.m[3 * i + 1], b.m[3 * j + 1],
+                                            a.m[3 * i + 2], b.m[3 * j + 2]);
+    const cd d = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+    return G[0][0].re + G[1][1].re + d.re;
+}
 // traceless anti-hermitian part: 1/2 (A - A^+) - 1/6 tr(A - A^+)
 __device__ __forceinline__ M3 algebra(const M3& a)
 {

@@ -256,7 +256,9 @@ __global__ void __launch_bounds__(BS) wchain_kernel(const double2* __restrict__ 
 }
 
 // the square loops of side L from the chains (wilson_loop.hpp:89-125), summed over
-// the six planes: Re tr (chain_mu(x) chain_nu(x + L mu) (chain_nu(x) chain_mu(x + L nu))^+)
+// the six planes: Re tr (chain_mu(x) chain_nu(x + L mu) (chain_nu(x) chain_mu(x + L nu))^+).
+// Both products are products of links, so in SU(3): rows 0 and 1 of each are multiplied
+// out and the trace is taken with retr_su3_nd_r01.
 __global__ void __launch_bounds__(BS) wloop_kernel(const double2* __restrict__ chain, Geom g, int L, double* part)
 {
     const int s = blockIdx.x * BS + threadIdx.x;
@@ -272,9 +274,9 @@ __global__ void __launch_bounds__(BS) wloop_kernel(const double2* __restrict__ c
         };
         for (int nu = 1; nu < 4; ++nu)
             for (int mu = 0; mu < nu; ++mu) {
-                const M3 A = mul(ld(chain, g, s, mu, par), ch(shift(g, s, c, mu, L % g.L[mu]), nu));
-                const M3 B = mul(ld(chain, g, s, nu, par), ch(shift(g, s, c, nu, L % g.L[nu]), mu));
-                w += retr_mul(A, dag(B));
+                const M3 A = mul_r01(ld(chain, g, s, mu, par), ch(shift(g, s, c, mu, L % g.L[mu]), nu));
+                const M3 B = mul_r01(ld(chain, g, s, nu, par), ch(shift(g, s, c, nu, L % g.L[nu]), mu));
+                w += retr_su3_nd_r01(A, B);
             }
     }
     block_sum2(w, 0.0, part + 2 * (size_t)blockIdx.x);
