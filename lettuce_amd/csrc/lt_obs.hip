@@ -124,22 +124,20 @@ qdensity_kernel(const double2* __restrict__ U, Geom g, double* __restrict__ q_x)
     q_x[s] = (-1.0 / (256.0 * M_PI * M_PI)) * (q01 - q02 + q03);
 }
 
-// one thread per timeslice: ordered sum over pairs and chunks
-__global__ void obs_finish_kernel(const double* part, int nchunk, int nt, double* out)
+// one block per timeslice: lane i sums the (pair, chunk) entries e = i, i + BS, ... of
+// its timeslice (e = pair * nchunk + chunk), then the block tree of block_sum3
+__global__ void __launch_bounds__(BS) obs_finish_kernel(const double* part, int nchunk, int nt, double* out)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
+    const int t = blockIdx.x;
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int pr = 0; pr < 3; ++pr)
-        for (int k = 0; k < nchunk; ++k) {
-            const double* p = part + 3 * (((size_t)pr * nt + t) * nchunk + k);
-            a += p[0];
-            b += p[1];
-            c += p[2];
-        }
-    out[3 * t] = a;
-    out[3 * t + 1] = b;
-    out[3 * t + 2] = c;
+    for (int e = threadIdx.x; e < 3 * nchunk; e += BS) {
+        const int pr = e / nchunk, k = e - pr * nchunk;
+        const double* p = part + 3 * (((size_t)pr * nt + t) * nchunk + k);
+        a += p[0];
+        b += p[1];
+        c += p[2];
+    }
+    block_sum3(a, b, c, out + 3 * (size_t)t);
 }
 
 // ---- the rest of Observables() (SU3_main.cpp:125-136) --------------------------
@@ -362,8 +360,8 @@ void k_observables(lt_field u, double* host_out)
     double* fin = part + 9 * (size_t)nchunk * nt;
     launch(ctx, "observables", obs_kernel, dim3(3 * nchunk * nt), dim3(BS), LDS_BYTES, (const double2*)u->d, g, Vs,
            (int)nchunk, part);
-    launch(ctx, "observables_finish", obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, (const double*)part,
-           (int)nchunk, nt, fin);
+    launch(ctx, "observables_finish", obs_finish_kernel, dim3(nt), dim3(BS), 0, (const double*)part, (int)nchunk, nt,
+           fin);
     LT_HIP_CHECK(hipMemcpyAsync(host_out, fin, 3 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
@@ -452,8 +450,8 @@ void k_observables_noalloc(lt_field u, double* host_out)
     hipLaunchKernelGGL(obs_kernel, dim3(3 * nchunk * nt), dim3(BS), LDS_BYTES, ctx->stream, (const double2*)u->d, g, Vs,
                        (int)nchunk, part);
     LT_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(obs_finish_kernel, dim3(nblocks(nt, 64)), dim3(64), 0, ctx->stream, (const double*)part,
-                       (int)nchunk, nt, fin);
+    hipLaunchKernelGGL(obs_finish_kernel, dim3(nt), dim3(BS), 0, ctx->stream, (const double*)part, (int)nchunk, nt,
+                       fin);
     LT_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(obs_final_kernel, dim3(1), dim3(64), 0, ctx->stream, (const double*)fin, nt, (double)g.V, out);
     LT_HIP_CHECK(hipGetLastError());

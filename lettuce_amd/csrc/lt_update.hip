@@ -168,18 +168,28 @@ __global__ void LT_LINK_KERNEL upd_plaq_kernel(const double2* __restrict__ U, Ge
     plaq_link_partials(U, g, m, part);
 }
 unsigned plaq_parts(const LinkMap& m) { return 8u * map_blocks(m); }
+// the sum of part[0, n) over one block, in a fixed order: lane i keeps eight running sums
+// of the entries i + RBS u (u < 8) of each group of 8 RBS entries (eight loads in flight
+// instead of one), the rest into the first, then a fixed pairing and the block tree
+__device__ __forceinline__ double partials_sum(const double* __restrict__ part, int n)
+{
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int i = threadIdx.x;
+    for (; i + 7 * RBS < n; i += 8 * RBS)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += part[i + u * RBS];
+    for (; i < n; i += RBS) a[0] += part[i];
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
 __global__ void __launch_bounds__(RBS) plaq_sum_kernel(const double* __restrict__ part, int n, double* out)
 {
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += RBS) acc += part[i];
-    block_sum1(acc, out);
+    block_sum1(partials_sum(part, n), out);
 }
+// the same sum into the plaquette slot of a captured update
 __global__ void __launch_bounds__(RBS) upd_plaq_finish_kernel(const double* __restrict__ part, int n,
                                                               const uint64_t* __restrict__ state, double* plaq)
 {
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += RBS) acc += part[i];
-    block_sum1(acc, plaq + state[1]);
+    block_sum1(partials_sum(part, n), plaq + state[1]);
 }
 // end of a captured update: the next one draws n_hb counters later and writes the next slot
 __global__ void upd_advance_kernel(uint64_t* state, int n_hb)
