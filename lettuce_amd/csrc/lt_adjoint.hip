@@ -26,11 +26,14 @@ __global__ void LT_LINK_KERNEL stout_tape_kernel(const double2* __restrict__ Uin
     body_stout_tape(Uin, Uout, T, g, m, link_block(m, blockIdx.x, gridDim.x), rho);
 }
 
-// one thread per (site, plane): the clover term of that plane
+// one thread per (site, plane): the clover term of that plane.  Block b (XCD-remapped)
+// takes plane b % 6 of site chunk b / 6, so the six planes of a chunk run in neighbouring
+// blocks of one XCD and share its L2 instead of streaming the field once per plane.
 __global__ void __launch_bounds__(CBS) clover_store_kernel(const double2* __restrict__ U, double2* __restrict__ C, Geom g)
 {
-    const int s = blockIdx.x * CBS + threadIdx.x;
-    if (s < g.V) body_clover_store(U, C, g, s, blockIdx.y);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int s = (b / 6) * CBS + threadIdx.x;
+    if (s < g.V) body_clover_store(U, C, g, s, b % 6);
 }
 
 // per site charge density, block partial sums
@@ -81,7 +84,7 @@ void k_stout_tape(lt_ctx ctx, const Geom& g, const LinkMap& m, const double2* in
 
 double k_clover_charge(lt_ctx ctx, const Geom& g, const double2* U, double2* C)
 {
-    launch(ctx, "clover_store", clover_store_kernel, dim3(nblocks(g.V, CBS), 6), dim3(CBS), 0, U, C, g);
+    launch(ctx, "clover_store", clover_store_kernel, dim3(6 * nblocks(g.V, CBS)), dim3(CBS), 0, U, C, g);
     unsigned nb = nblocks(g.V, CBS);
     double* part = ctx->scratch_for(nb + 1);
     launch(ctx, "qdensity", qdensity_kernel, dim3(nb), dim3(CBS), 0, (const double2*)C, g, part);
