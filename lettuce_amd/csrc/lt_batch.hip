@@ -125,13 +125,17 @@ __global__ void __launch_bounds__(RBS) bk_copy(const double2* __restrict__ src, 
     }
 }
 
+// block b (XCD-remapped) takes plane b % 6 of site block b / 6: the six planes of a site
+// block in neighbouring blocks of one XCD
 __global__ void __launch_bounds__(RBS) bk_clover_store(const double2* __restrict__ U, double2* __restrict__ C, BatchGrid bg)
 {
-    const int r = rep_of(bg.sb, bg.n, blockIdx.x);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int sbk = b / 6;
+    const int r = rep_of(bg.sb, bg.n, sbk);
     const Geom g = bg.reps[r].g;
     const long long off = bg.reps[r].off;
-    const int s = (blockIdx.x - bg.sb[r]) * RBS + threadIdx.x;
-    if (s < g.V) body_clover_store(U + off, C + clover_offset(off), g, s, blockIdx.y);
+    const int s = (sbk - bg.sb[r]) * RBS + threadIdx.x;
+    if (s < g.V) body_clover_store(U + off, C + clover_offset(off), g, s, b % 6);
 }
 
 __global__ void __launch_bounds__(RBS) bk_qdensity(const double2* __restrict__ C, BatchGrid bg, double* part)
@@ -385,7 +389,7 @@ void kb_stout_tape(lt_ctx ctx, const BatchGrid& bg, int total_lb, const double2*
 void kb_clover_charge(lt_ctx ctx, const BatchGrid& bg, int total_sb, const double2* U, double2* C, double* part,
                       double* qraw)
 {
-    launch(ctx, "batch_clover_store", bk_clover_store, dim3(total_sb, 6), dim3(RBS), 0, U, C, bg);
+    launch(ctx, "batch_clover_store", bk_clover_store, dim3(6 * total_sb), dim3(RBS), 0, U, C, bg);
     launch(ctx, "batch_qdensity", bk_qdensity, dim3(total_sb), dim3(RBS), 0, (const double2*)C, bg, part);
     launch(ctx, "batch_finish", bk_finish, dim3(bg.n), dim3(RBS), 0, (const double*)part, bg.sb, qraw);
 }
